@@ -1,0 +1,157 @@
+/*
+ * qtx.h -- C ABI of the MI355X (gfx950) S3TC block codec.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point below
+ * replaces one call that the reference's pybind11 extension `_quicktex` makes
+ * into its C++ codecs; the replaced interface is cited beside each one
+ * (paths relative to the reference tree).
+ *
+ * Conventions
+ *   - Raw textures are row-major RGBA8, width*height*4 bytes, no row padding
+ *     (quicktex/Texture.h:73,87).
+ *   - Block textures are row-major blocks, ceil(w/4) x ceil(h/4), 8 bytes per
+ *     BC1/BC4 block and 16 bytes per BC3/BC5 block (quicktex/Texture.h:158-162).
+ *   - `*_device` entry points take device pointers and a hipStream_t passed as
+ *     `void*` (NULL = the null stream); they enqueue work and return without
+ *     synchronising.  The host entry points copy host buffers through the
+ *     context's staging memory and return after the result is back on the host.
+ *   - Nothing is thrown across the ABI.  On failure a status is returned and
+ *     qtx_last_error() (thread-local) describes it.
+ */
+#ifndef QTX_H
+#define QTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum qtx_status {
+    QTX_OK = 0,
+    QTX_EINVAL = 1,  /* bad argument: maps to Python ValueError      */
+    QTX_ERANGE = 2,  /* index/size out of range: maps to IndexError  */
+    QTX_ENOMEM = 3,  /* host or device allocation failed             */
+    QTX_EDEVICE = 4, /* HIP runtime error                            */
+    QTX_EBUSY = 5,   /* serving queue full / request not finished    */
+    QTX_ENOTSUP = 6  /* request the library cannot serve             */
+} qtx_status;
+
+/* Interpolator::Type (quicktex/s3tc/interpolator/Interpolator.h:30) */
+enum { QTX_INTERP_IDEAL = 0, QTX_INTERP_IDEAL_ROUND = 1, QTX_INTERP_NVIDIA = 2, QTX_INTERP_AMD = 3 };
+/* BC1Encoder::ColorMode (quicktex/s3tc/bc1/BC1Encoder.h:51-65) */
+enum { QTX_COLOR_FOUR = 4, QTX_COLOR_THREE = 3, QTX_COLOR_THREE_BLACK = 0x13 };
+/* BC1Encoder::ErrorMode (quicktex/s3tc/bc1/BC1Encoder.h:67-79) */
+enum { QTX_ERROR_NONE = 0, QTX_ERROR_FASTER = 1, QTX_ERROR_CHECK2 = 2, QTX_ERROR_FULL = 3 };
+/* BC1Encoder::EndpointMode (quicktex/s3tc/bc1/BC1Encoder.h:81-94) */
+enum { QTX_EP_LEAST_SQUARES = 0, QTX_EP_BOUNDING_BOX = 1, QTX_EP_BOUNDING_BOX_INT = 2, QTX_EP_PCA = 3 };
+/* Block formats for the batched and served forms */
+enum { QTX_FMT_BC1 = 1, QTX_FMT_BC3 = 3, QTX_FMT_BC4 = 4, QTX_FMT_BC5 = 5 };
+
+/*
+ * Complete BC1 encoder state after the reference's clamps: the fields of
+ * BC1Encoder that EncodeBlock reads (quicktex/s3tc/bc1/BC1Encoder.h:96-99,
+ * 151-168).  Fill it with qtx_bc1_params_init() and then, like the Python
+ * advanced API, overwrite single fields.
+ */
+typedef struct qtx_bc1_params {
+    int32_t interpolator;      /* QTX_INTERP_*                                 */
+    int32_t color_mode;        /* QTX_COLOR_*                                  */
+    int32_t error_mode;        /* QTX_ERROR_*                                  */
+    int32_t endpoint_mode;     /* QTX_EP_*                                     */
+    int32_t two_ls_passes;     /* bool                                         */
+    int32_t two_ep_passes;     /* bool                                         */
+    int32_t two_cf_passes;     /* bool                                         */
+    int32_t exhaustive;        /* bool                                         */
+    uint32_t power_iterations; /* clamped to [4, 10]                           */
+    uint32_t search_rounds;    /* any value                                    */
+    uint32_t orderings4;       /* clamped to [1, 128]                          */
+    uint32_t orderings3;       /* clamped to [1, 32]                           */
+} qtx_bc1_params;
+
+/* BC1Encoder(level, color_mode, interpolator) ctor + SetLevel:
+ * quicktex/s3tc/bc1/BC1Encoder.cpp:47-249.  EINVAL for level > 19, a colour
+ * mode other than 4/3/0x13, or an unknown interpolator. */
+qtx_status qtx_bc1_params_init(qtx_bc1_params *out, uint32_t level, int32_t color_mode, int32_t interpolator);
+/* Re-applies the clamps of BC1Encoder::SetOrderings4/3 and
+ * SetPowerIterations (quicktex/s3tc/bc1/BC1Encoder.cpp:251-258) and validates
+ * the enums.  Every encode entry point calls it on a private copy. */
+qtx_status qtx_bc1_params_clamp(qtx_bc1_params *p);
+
+/* Thread-local description of the last failure on this thread. */
+const char *qtx_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Context: one device, its stream, device-resident tables and staging.    */
+typedef struct qtx_ctx qtx_ctx;
+qtx_status qtx_ctx_create(int32_t device, qtx_ctx **out);
+qtx_status qtx_ctx_destroy(qtx_ctx *ctx);
+/* Blocks until all work the context enqueued on its own stream is done. */
+qtx_status qtx_ctx_synchronize(qtx_ctx *ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident entry points (inputs and outputs already in HBM).       */
+
+/* BlockEncoder<BlockTexture<BC1Block>>::Encode with BC1Encoder::EncodeBlock
+ * (quicktex/Encoder.h:36-56, quicktex/s3tc/bc1/BC1Encoder.cpp:261-342). */
+qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream);
+/* BC3Encoder::Encode (quicktex/s3tc/bc3/BC3Encoder.cpp:27-32): BC4 of
+ * channel 3 then BC1; params->color_mode must be QTX_COLOR_FOUR. */
+qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream);
+/* BC4Encoder(channel).Encode (quicktex/s3tc/bc4/BC4Encoder.cpp:30-71). */
+qtx_status qtx_bc4_encode_device(qtx_ctx *ctx, uint32_t channel, const uint8_t *d_rgba, int32_t width, int32_t height,
+                                 uint8_t *d_blocks, void *stream);
+/* BC5Encoder(chan0, chan1).Encode (quicktex/s3tc/bc5/BC5Encoder.cpp:25-30). */
+qtx_status qtx_bc5_encode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream);
+/* BC1Decoder(write_alpha, interpolator).Decode
+ * (quicktex/Decoder.h:36-54, quicktex/s3tc/bc1/BC1Decoder.cpp:31-52). */
+qtx_status qtx_bc1_decode_device(qtx_ctx *ctx, int32_t interpolator, int32_t write_alpha, const uint8_t *d_blocks,
+                                 int32_t width, int32_t height, uint8_t *d_rgba, void *stream);
+/* BC3Decoder(interpolator).Decode (quicktex/s3tc/bc3/BC3Decoder.cpp:26-32). */
+qtx_status qtx_bc3_decode_device(qtx_ctx *ctx, int32_t interpolator, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream);
+/* BC4Decoder(channel).Decode (quicktex/s3tc/bc4/BC4Decoder.cpp:29-49). */
+qtx_status qtx_bc4_decode_device(qtx_ctx *ctx, uint32_t channel, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream);
+/* BC5Decoder(chan0, chan1).Decode (quicktex/s3tc/bc5/BC5Decoder.cpp:25-30). */
+qtx_status qtx_bc5_decode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Host entry points: what the pybind11 layer calls for encoder.encode()   */
+/* and decoder.decode().  `out_bytes` must be at least the texture size.   */
+
+/* BC1Encoder.encode (quicktex/s3tc/bc1/_bindings.cpp:113). */
+qtx_status qtx_bc1_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes);
+/* BC3Encoder.encode (quicktex/s3tc/bc3/_bindings.cpp:83). */
+qtx_status qtx_bc3_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes);
+/* BC4Encoder.encode (quicktex/s3tc/bc4/_bindings.cpp:92). */
+qtx_status qtx_bc4_encode(qtx_ctx *ctx, uint32_t channel, const uint8_t *rgba, int32_t width, int32_t height,
+                          uint8_t *out, size_t out_bytes);
+/* BC5Encoder.encode (quicktex/s3tc/bc5/_bindings.cpp:74). */
+qtx_status qtx_bc5_encode(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes);
+/* BC1Decoder.decode (quicktex/s3tc/bc1/_bindings.cpp:180). */
+qtx_status qtx_bc1_decode(qtx_ctx *ctx, int32_t interpolator, int32_t write_alpha, const uint8_t *blocks,
+                          int32_t width, int32_t height, uint8_t *rgba_out, size_t out_bytes);
+/* BC3Decoder.decode (quicktex/s3tc/bc3/_bindings.cpp:108). */
+qtx_status qtx_bc3_decode(qtx_ctx *ctx, int32_t interpolator, const uint8_t *blocks, int32_t width, int32_t height,
+                          uint8_t *rgba_out, size_t out_bytes);
+/* BC4Decoder.decode (quicktex/s3tc/bc4/_bindings.cpp:113). */
+qtx_status qtx_bc4_decode(qtx_ctx *ctx, uint32_t channel, const uint8_t *blocks, int32_t width, int32_t height,
+                          uint8_t *rgba_out, size_t out_bytes);
+/* BC5Decoder.decode (quicktex/s3tc/bc5/_bindings.cpp:98). */
+qtx_status qtx_bc5_decode(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *blocks, int32_t width,
+                          int32_t height, uint8_t *rgba_out, size_t out_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QTX_H */

@@ -1,0 +1,135 @@
+"""ctypes loaders for the CPU checkers (test infrastructure only).
+
+* ``oracle()``: oracle/liboracle.so, the CPU restatement of the reference path.
+* ``reference()``: oracle/_ref/libqtxref.so, the reference's own C++ compiled
+  from its sources (built only where the reference tree exists; the built
+  library travels with the repository snapshot).  None when absent.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class BC1Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        'interpolator', 'color_mode', 'error_mode', 'endpoint_mode',
+        'two_ls_passes', 'two_ep_passes', 'two_cf_passes', 'exhaustive')] + \
+        [(n, ctypes.c_uint32) for n in ('power_iterations', 'search_rounds', 'orderings4', 'orderings3')]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_U8P)
+
+
+class _Codec:
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.p = prefix
+        for name in ('bc1_encode', 'bc3_encode'):
+            getattr(self.lib, prefix + name).argtypes = [ctypes.POINTER(BC1Params), _U8P, ctypes.c_int32, ctypes.c_int32, _U8P]
+        err = getattr(self.lib, prefix + 'last_error')
+        err.restype = ctypes.c_char_p
+        self._err = err
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, self.p + name)(*args)
+        if rc != 0:
+            raise ValueError(self._err().decode())
+
+    def params(self, level=5, color_mode=4, interpolator=0):
+        p = BC1Params()
+        self._call('bc1_params_init', ctypes.byref(p), ctypes.c_uint32(level), ctypes.c_int32(color_mode),
+                   ctypes.c_int32(interpolator))
+        return p
+
+    @staticmethod
+    def _nblocks(w, h):
+        return ((w + 3) // 4) * ((h + 3) // 4)
+
+    def bc1_encode(self, rgba, params):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(self._nblocks(w, h) * 8, np.uint8)
+        self._call('bc1_encode', ctypes.byref(params), _ptr(rgba), w, h, _ptr(out))
+        return out
+
+    def bc3_encode(self, rgba, params):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(self._nblocks(w, h) * 16, np.uint8)
+        self._call('bc3_encode', ctypes.byref(params), _ptr(rgba), w, h, _ptr(out))
+        return out
+
+    def bc4_encode(self, rgba, channel=3):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(self._nblocks(w, h) * 8, np.uint8)
+        self._call('bc4_encode', ctypes.c_uint32(channel), _ptr(rgba), w, h, _ptr(out))
+        return out
+
+    def bc5_encode(self, rgba, chan0=0, chan1=1):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(self._nblocks(w, h) * 16, np.uint8)
+        self._call('bc5_encode', ctypes.c_uint32(chan0), ctypes.c_uint32(chan1), _ptr(rgba), w, h, _ptr(out))
+        return out
+
+    def bc1_decode(self, blocks, w, h, interpolator=0, write_alpha=False):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self._call('bc1_decode', ctypes.c_int32(interpolator), ctypes.c_int32(int(write_alpha)), _ptr(blocks), w, h, _ptr(out))
+        return out
+
+    def bc3_decode(self, blocks, w, h, interpolator=0):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self._call('bc3_decode', ctypes.c_int32(interpolator), _ptr(blocks), w, h, _ptr(out))
+        return out
+
+    def bc4_decode(self, blocks, w, h, channel=3):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self._call('bc4_decode', ctypes.c_uint32(channel), _ptr(blocks), w, h, _ptr(out))
+        return out
+
+    def bc5_decode(self, blocks, w, h, chan0=0, chan1=1):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self._call('bc5_decode', ctypes.c_uint32(chan0), ctypes.c_uint32(chan1), _ptr(blocks), w, h, _ptr(out))
+        return out
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_path():
+    return os.path.join(ROOT, 'oracle', 'liboracle.so')
+
+
+def reference_path():
+    return os.path.join(ROOT, 'oracle', '_ref', 'libqtxref.so')
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _oracle = _Codec(oracle_path(), 'qtxo_')
+        _oracle.lib.qtxo_set_threads.argtypes = [ctypes.c_int]
+    return _oracle
+
+
+def reference():
+    global _ref
+    if _ref is None and os.path.exists(reference_path()):
+        _ref = _Codec(reference_path(), 'qtxr_')
+    return _ref
