@@ -1,0 +1,421 @@
+// capi.cpp -- the C ABI (include/qtx.h) over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "kernels.h"
+#include "qtx_internal.h"
+
+struct qtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    qtx::DeviceTables *d_tables = nullptr;
+    // staging for the host entry points
+    uint8_t *d_in = nullptr;
+    size_t in_cap = 0;
+    uint8_t *d_out = nullptr;
+    size_t out_cap = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+qtx_status fail(qtx_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+qtx_status hip_fail(hipError_t e, const char *what) {
+    return fail(QTX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define QTX_HIP(call)                                         \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);     \
+    } while (0)
+
+qtx_status check_dims(int32_t w, int32_t h) {
+    // RawTexture/BlockTexture constructors (quicktex/Texture.h:56-57)
+    if (w <= 0) return fail(QTX_EINVAL, "Texture width must be greater than 0");
+    if (h <= 0) return fail(QTX_EINVAL, "Texture height must be greater than 0");
+    const long long bw = (w + 3LL) / 4, bh = (h + 3LL) / 4;
+    if ((long long)w * h > (1LL << 31) || bw * bh * 16 >= (1LL << 31))
+        return fail(QTX_ERANGE, "texture of %d x %d exceeds the 2^31-texel limit of this library", w, h);
+    return QTX_OK;
+}
+
+size_t block_bytes(int32_t w, int32_t h, int bytes_per_block) {
+    return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * (size_t)bytes_per_block;
+}
+
+qtx_status check_ctx(qtx_ctx *ctx) {
+    if (!ctx) return fail(QTX_EINVAL, "null context");
+    QTX_HIP(hipSetDevice(ctx->device));
+    return QTX_OK;
+}
+
+qtx_status to_settings(const qtx_bc1_params *p, qtx::BC1Settings &s, int &interp) {
+    if (!p) return fail(QTX_EINVAL, "null params");
+    qtx_bc1_params c = *p;
+    qtx_status st = qtx_bc1_params_clamp(&c);
+    if (st != QTX_OK) return st;
+    s.color_mode = c.color_mode;
+    s.error_mode = c.error_mode;
+    s.endpoint_mode = c.endpoint_mode;
+    s.ls_passes = c.two_ls_passes ? 2 : 1;
+    // orderings are >= 1, so every block needs its error and two_ep applies
+    // (BC1Encoder.cpp:269-282)
+    s.ep_passes = c.two_ep_passes ? 2 : 1;
+    s.cf_passes = c.two_cf_passes ? 2 : 1;
+    s.exhaustive = c.exhaustive ? 1 : 0;
+    s.power_iterations = c.power_iterations;
+    s.search_rounds = c.search_rounds;
+    s.orderings4 = c.orderings4;
+    s.orderings3 = c.orderings3;
+    interp = c.interpolator;
+    return QTX_OK;
+}
+
+qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need) {
+    if (need <= cap) return QTX_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    const size_t n = std::max(need, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&buf, n);
+    if (e != hipSuccess) return fail(QTX_ENOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+    cap = n;
+    return QTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qtx_last_error(void) { return g_err.c_str(); }
+
+qtx_status qtx_bc1_params_clamp(qtx_bc1_params *p) {
+    if (!p) return fail(QTX_EINVAL, "null params");
+    if (p->interpolator < 0 || p->interpolator > 3) return fail(QTX_EINVAL, "Invalid interpolator type");
+    if (p->color_mode != QTX_COLOR_FOUR && p->color_mode != QTX_COLOR_THREE && p->color_mode != QTX_COLOR_THREE_BLACK)
+        return fail(QTX_EINVAL, "Encoder color mode must be FourColor, ThreeColor, or ThreeColorBlack");
+    if (p->error_mode < 0 || p->error_mode > 3) return fail(QTX_EINVAL, "invalid error mode %d", p->error_mode);
+    if (p->endpoint_mode < 0 || p->endpoint_mode > 3) return fail(QTX_EINVAL, "invalid endpoint mode %d", p->endpoint_mode);
+    // BC1Encoder.cpp:251-258
+    p->orderings4 = std::min(std::max(p->orderings4, 1u), (uint32_t)qtx::kBestOrders4);
+    p->orderings3 = std::min(std::max(p->orderings3, 1u), (uint32_t)qtx::kBestOrders3);
+    p->power_iterations = std::min(std::max(p->power_iterations, 4u), 10u);
+    return QTX_OK;
+}
+
+qtx_status qtx_bc1_params_init(qtx_bc1_params *p, uint32_t level, int32_t color_mode, int32_t interpolator) {
+    if (!p) return fail(QTX_EINVAL, "null params");
+    if (color_mode != QTX_COLOR_FOUR && color_mode != QTX_COLOR_THREE && color_mode != QTX_COLOR_THREE_BLACK)
+        return fail(QTX_EINVAL, "Encoder color mode must be FourColor, ThreeColor, or ThreeColorBlack");
+    if (interpolator < 0 || interpolator > 3) return fail(QTX_EINVAL, "Invalid interpolator type");
+    if (level > 19) return fail(QTX_EINVAL, "Level out of range, bust be between 0 and 18 inclusive");
+    // Presets of BC1Encoder::SetLevel (quicktex/s3tc/bc1/BC1Encoder.cpp:74-249):
+    //   level: endpoint, error, power iterations, ls2, ep2, cf2, exhaustive, o4, o3, search rounds
+    struct Preset {
+        int8_t ep, err, pi, ls2, ep2, cf2, ex;
+        uint8_t o4, o3;
+        uint16_t search;
+    };
+    static const Preset presets[20] = {
+        {QTX_EP_BOUNDING_BOX_INT, QTX_ERROR_CHECK2, 4, 0, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_LEAST_SQUARES, QTX_ERROR_CHECK2, 4, 0, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 0, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 1, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_PCA, QTX_ERROR_FASTER, 4, 1, 0, 0, 0, 0, 0, 0},
+        {QTX_EP_PCA, QTX_ERROR_FASTER, 4, 1, 0, 0, 0, 1, 1, 0},
+        {QTX_EP_PCA, QTX_ERROR_FASTER, 4, 1, 0, 0, 0, 4, 1, 0},
+        {QTX_EP_PCA, QTX_ERROR_FASTER, 4, 1, 0, 0, 0, 8, 1, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 1, 0, 0, 0, 11, 3, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 1, 0, 0, 0, 20, 8, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 1, 0, 0, 0, 28, 16, 0},
+        {QTX_EP_PCA, QTX_ERROR_CHECK2, 4, 1, 0, 0, 0, 32, 32, 0},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 0, 0, 32, 32, 20},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 0, 0, 32, 32, 32},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 0, 0, 56, 32, 32},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 0, 0, 80, 32, 256},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 4, 1, 1, 0, 0, 128, 32, 256},  // level 17 leaves power iterations at 4
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 1, 0, 128, 32, 256},
+        {QTX_EP_PCA, QTX_ERROR_FULL, 6, 1, 1, 1, 1, 128, 32, 256},
+    };
+    const Preset &s = presets[level];
+    p->interpolator = interpolator;
+    p->color_mode = color_mode;
+    p->error_mode = s.err;
+    p->endpoint_mode = s.ep;
+    p->two_ls_passes = s.ls2;
+    p->two_ep_passes = s.ep2;
+    p->two_cf_passes = s.cf2;
+    p->exhaustive = s.ex;
+    p->power_iterations = (uint32_t)s.pi;
+    p->search_rounds = s.search;
+    p->orderings4 = s.o4;
+    p->orderings3 = s.o3;
+    return qtx_bc1_params_clamp(p);
+}
+
+qtx_status qtx_ctx_create(int32_t device, qtx_ctx **out) {
+    if (!out) return fail(QTX_EINVAL, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    QTX_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(QTX_EINVAL, "device %d out of range (%d devices)", device, n);
+    QTX_HIP(hipSetDevice(device));
+    auto *ctx = new qtx_ctx;
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_tables, sizeof(qtx::DeviceTables));
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->d_tables, &qtx::host_tables(), sizeof(qtx::DeviceTables), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        qtx_ctx_destroy(ctx);
+        return hip_fail(e, "context set-up");
+    }
+    *out = ctx;
+    return QTX_OK;
+}
+
+qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
+    if (!ctx) return QTX_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_in) (void)hipFree(ctx->d_in);
+    if (ctx->d_out) (void)hipFree(ctx->d_out);
+    if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return QTX_OK;
+}
+
+qtx_status qtx_ctx_synchronize(qtx_ctx *ctx) {
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    QTX_HIP(hipStreamSynchronize(ctx->stream));
+    return QTX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident entry points
+
+qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    qtx::BC1Settings s;
+    int interp = 0;
+    if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false,
+                                   (hipStream_t)stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    qtx::BC1Settings s;
+    int interp = 0;
+    if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
+    if (s.color_mode != QTX_COLOR_FOUR) return fail(QTX_EINVAL, "BC3 colour blocks are always FourColor");
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, true,
+                                   (hipStream_t)stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_bc4_encode_device(qtx_ctx *ctx, uint32_t channel, const uint8_t *d_rgba, int32_t width, int32_t height,
+                                 uint8_t *d_blocks, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (channel >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QTX_HIP(qtx::launch_bc4_encode(d_rgba, width, height, d_blocks, (int)channel, 0, false, (hipStream_t)stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_bc5_encode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *d_rgba, int32_t width,
+                                 int32_t height, uint8_t *d_blocks, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (chan0 >= 4 || chan1 >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QTX_HIP(qtx::launch_bc4_encode(d_rgba, width, height, d_blocks, (int)chan0, (int)chan1, true, (hipStream_t)stream));
+    return QTX_OK;
+}
+
+static qtx_status decode_device(qtx_ctx *ctx, int format, int interp, int p0, int p1, const uint8_t *d_blocks,
+                                int32_t width, int32_t height, uint8_t *d_rgba, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (interp < 0 || interp > 3) return fail(QTX_EINVAL, "Invalid interpolator type");
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QTX_HIP(qtx::launch_decode(format, d_blocks, width, height, d_rgba, interp, p0, p1, (hipStream_t)stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_bc1_decode_device(qtx_ctx *ctx, int32_t interpolator, int32_t write_alpha, const uint8_t *d_blocks,
+                                 int32_t width, int32_t height, uint8_t *d_rgba, void *stream) {
+    return decode_device(ctx, QTX_FMT_BC1, interpolator, write_alpha != 0, 0, d_blocks, width, height, d_rgba, stream);
+}
+
+qtx_status qtx_bc3_decode_device(qtx_ctx *ctx, int32_t interpolator, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream) {
+    return decode_device(ctx, QTX_FMT_BC3, interpolator, 0, 0, d_blocks, width, height, d_rgba, stream);
+}
+
+qtx_status qtx_bc4_decode_device(qtx_ctx *ctx, uint32_t channel, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream) {
+    if (channel >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return decode_device(ctx, QTX_FMT_BC4, 0, (int)channel, 0, d_blocks, width, height, d_rgba, stream);
+}
+
+qtx_status qtx_bc5_decode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *d_blocks, int32_t width,
+                                 int32_t height, uint8_t *d_rgba, void *stream) {
+    if (chan0 >= 4 || chan1 >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return decode_device(ctx, QTX_FMT_BC5, 0, (int)chan0, (int)chan1, d_blocks, width, height, d_rgba, stream);
+}
+
+// ---------------------------------------------------------------------------
+// host entry points: stage through the context's device buffers
+
+}  // extern "C"
+
+namespace {
+
+template <class Launch>
+qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *out, size_t out_need, size_t out_bytes,
+                     Launch &&launch) {
+    if (!ctx) return fail(QTX_EINVAL, "null context");
+    if (!in || !out) return fail(QTX_EINVAL, "null buffer");
+    if (out_bytes < out_need) return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", out_bytes, out_need);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    QTX_HIP(hipSetDevice(ctx->device));
+    qtx_status st = ensure(ctx->d_in, ctx->in_cap, in_bytes);
+    if (st == QTX_OK) st = ensure(ctx->d_out, ctx->out_cap, out_need);
+    if (st != QTX_OK) return st;
+    QTX_HIP(hipMemcpyAsync(ctx->d_in, in, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    st = launch(ctx->d_in, ctx->d_out, (void *)ctx->stream);
+    if (st != QTX_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st;
+    }
+    QTX_HIP(hipMemcpyAsync(out, ctx->d_out, out_need, hipMemcpyDeviceToHost, ctx->stream));
+    QTX_HIP(hipStreamSynchronize(ctx->stream));
+    return QTX_OK;
+}
+
+size_t raw_bytes(int32_t w, int32_t h) { return (size_t)w * (size_t)h * 4; }
+
+}  // namespace
+
+extern "C" {
+
+qtx_status qtx_bc1_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 8), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc1_encode_device(ctx, params, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc3_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 16), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc3_encode_device(ctx, params, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc4_encode(qtx_ctx *ctx, uint32_t channel, const uint8_t *rgba, int32_t width, int32_t height,
+                          uint8_t *out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (channel >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 8), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc4_encode_device(ctx, channel, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc5_encode(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *rgba, int32_t width,
+                          int32_t height, uint8_t *out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (chan0 >= 4 || chan1 >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 16), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc5_encode_device(ctx, chan0, chan1, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc1_decode(qtx_ctx *ctx, int32_t interpolator, int32_t write_alpha, const uint8_t *blocks,
+                          int32_t width, int32_t height, uint8_t *rgba_out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    return host_call(ctx, blocks, block_bytes(width, height, 8), rgba_out, raw_bytes(width, height), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc1_decode_device(ctx, interpolator, write_alpha, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc3_decode(qtx_ctx *ctx, int32_t interpolator, const uint8_t *blocks, int32_t width, int32_t height,
+                          uint8_t *rgba_out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    return host_call(ctx, blocks, block_bytes(width, height, 16), rgba_out, raw_bytes(width, height), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc3_decode_device(ctx, interpolator, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc4_decode(qtx_ctx *ctx, uint32_t channel, const uint8_t *blocks, int32_t width, int32_t height,
+                          uint8_t *rgba_out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (channel >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return host_call(ctx, blocks, block_bytes(width, height, 8), rgba_out, raw_bytes(width, height), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc4_decode_device(ctx, channel, di, width, height, dout, s);
+                     });
+}
+
+qtx_status qtx_bc5_decode(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *blocks, int32_t width,
+                          int32_t height, uint8_t *rgba_out, size_t out_bytes) {
+    qtx_status st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    if (chan0 >= 4 || chan1 >= 4) return fail(QTX_EINVAL, "Channel out of range");
+    return host_call(ctx, blocks, block_bytes(width, height, 16), rgba_out, raw_bytes(width, height), out_bytes,
+                     [&](const uint8_t *di, uint8_t *dout, void *s) {
+                         return qtx_bc5_decode_device(ctx, chan0, chan1, di, width, height, dout, s);
+                     });
+}
+
+}  // extern "C"

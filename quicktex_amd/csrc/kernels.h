@@ -1,0 +1,17 @@
+// kernels.h -- launchers of the gfx950 kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qtx_internal.h"
+
+namespace qtx {
+
+hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
+                             const BC1Settings &s, int interp, bool bc3, hipStream_t stream);
+hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, int ch0, int ch1, bool bc5,
+                             hipStream_t stream);
+// format: QTX_FMT_*; p0/p1: write_alpha for BC1, channels for BC4/BC5
+hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint8_t *d_rgba, int interp, int p0, int p1,
+                         hipStream_t stream);
+
+}  // namespace qtx
