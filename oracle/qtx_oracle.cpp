@@ -35,6 +35,42 @@
 namespace {
 
 // ----------------------------------------------------------------------------
+// Algorithmic work accounting.  Each primitive of the encoder counts its
+// invocations; qtxo_op_weights() gives the static number of scalar integer /
+// FP32 operations one invocation of a straightforward implementation needs
+// (16 texels, the reference's formulation).  ops = sum(count * weight) is the
+// "algorithmic ops" of a workload.  Counts are per thread: count single-threaded.
+enum OpKind {
+    OP_METRICS, OP_SOLID, OP_EP_GREY, OP_EP_LS, OP_EP_BB, OP_EP_BBINT, OP_EP_PCA, OP_PCA_ITER,
+    OP_SEL_FULL4, OP_SEL_FULL3, OP_SEL_CHECK2, OP_SEL_FASTER, OP_SEL_NONE,
+    OP_LS_PX, OP_LS_SUMS, OP_SC_PX, OP_CF_SETUP, OP_SEARCH_ROUND, OP_WRITE, OP_BC4, OP_KINDS
+};
+const uint32_t kOpWeight[OP_KINDS] = {
+    280,  // metrics: per texel min/max/sum/grey/black tests
+    30,   // solid block: 3 table lookups, packing, order fix-up
+    15,   // greyscale endpoints
+    146,  // least-squares endpoints: 16 x (3 mul + 3 add) + solve
+    136,  // bounding box (float): insets, 16 x covariance sign terms
+    121,  // bounding box (int)
+    421,  // PCA: covariance 16 x 15, scaling, 16 projections x 10
+    32,   // one PCA power iteration: 4 rows x (4 mul + 4 add)
+    702,  // 4-candidate full selector search: palette + 16 x (4 x 10 + 2)
+    542,  // 3-candidate full selector search
+    568,  // Check2 selector search: 16 x 33 + setup
+    365,  // Faster selector search: 16 x 20 + setup
+    237,  // selector search without error
+    226,  // least squares from texels: 16 x 10 + solve + 2 roundings
+    70,   // least squares from prefix sums + 2 roundings
+    216,  // single-colour fallback error: palette + 16 x 11
+    310,  // cluster-fit set-up: histogram, 16 projections, sort, prefix sums
+    8,    // one endpoint-search move
+    42,   // block packing
+    180,  // one BC4 channel block
+};
+thread_local uint64_t g_ops[OP_KINDS];
+inline void count(OpKind k, uint64_t n = 1) { g_ops[k] += n; }
+
+// ----------------------------------------------------------------------------
 // Colour helpers: quicktex/Color.h:29-96, quicktex/Color.cpp, quicktex/util.h
 
 struct C4 {
@@ -223,16 +259,14 @@ void build_orders(Orders &o, int n) {
         o.weights[s][2] = (float)(s * (d - s));
         o.weights[s][3] = (float)((d - s) * (d - s));
     }
+    // SingleColorHashes (OrderTable.cpp:43-44) are data: the 3-colour list maps
+    // bins 1 and 2 crosswise, and GetHash (OrderTable.h:91-104) indexes it by bin.
+    for (int s = 0; s < n; s++) o.single_color_hash[s] = n == 4 ? qtx_single_color_hashes4[s] : qtx_single_color_hashes3[s];
     const float denominator = n == 4 ? 3.0f : 2.0f;
     for (int i = 0; i < o.count; i++) {
         const uint8_t *h = o.hist + i * n;
         bool any16 = false;
-        for (int s = 0; s < n; s++) {
-            if (h[s] == 16) {
-                any16 = true;
-                o.single_color_hash[s] = (uint16_t)i;
-            }
-        }
+        for (int s = 0; s < n; s++) any16 |= h[s] == 16;
         if (!any16) o.hashes[packed_hist(h, n)] = (uint16_t)i;
         float m[4] = {0, 0, 0, 0};
         for (int s = 0; s < n; s++)
@@ -293,6 +327,7 @@ bool is_single_color(const Block &b) {
 }
 
 Metrics get_metrics(const Block &b, bool ignore_black) {
+    count(OP_METRICS);
     Metrics m;
     m.min = mk(255, 255, 255);
     m.max = mk(0, 0, 0);
@@ -354,6 +389,7 @@ void endpoints_single_color(const Enc &e, Result &r, const C4 &color, bool is_3c
 // The error is measured against colour 2 of the endpoints the result held on
 // entry, with alpha included.
 void endpoints_single_color_px(const Enc &e, Result &r, const Block &b, const C4 &color, bool is_3color) {
+    count(OP_SC_PX);
     C4 pal[4];
     interpolate_bc1(e.p.interpolator, r.low, r.high, is_3color, pal);
     const C4 ref = pal[2];
@@ -373,6 +409,7 @@ void endpoints_single_color_px(const Enc &e, Result &r, const Block &b, const C4
 // FindEndpoints: BC1Encoder.cpp:473-677
 void find_endpoints(const Enc &e, Result &r, const Block &b, const Metrics &m, int endpoint_mode, bool ignore_black) {
     if (m.is_greyscale) {
+        count(OP_EP_GREY);
         const int fr = b.px[0].v[0];  // pixel (0,0) even when ignoring black (BC1Encoder.cpp:476)
         if ((int)m.max.v[0] - (int)m.min.v[0] < 2) {
             const int f5 = scale8to5(fr), f6 = scale8to6(fr);
@@ -383,6 +420,7 @@ void find_endpoints(const Enc &e, Result &r, const Block &b, const Metrics &m, i
             r.high = mk(scale8to5(m.max.v[0]), scale8to6(m.max.v[0]), scale8to5(m.max.v[0]));
         }
     } else if (endpoint_mode == QTX_EP_LEAST_SQUARES) {
+        count(OP_EP_LS);
         // BC1Encoder.cpp:495-552.  The reference leaves sums_xy uninitialised
         // (:510); it is zero here.  Both endpoints are placed at min on the primary
         // axis (:505-506), so low == high whatever the accumulator holds.
@@ -419,6 +457,7 @@ void find_endpoints(const Enc &e, Result &r, const Block &b, const Metrics &m, i
         r.low = precise_round565(l);
         r.high = precise_round565(h);
     } else if (endpoint_mode == QTX_EP_BOUNDING_BOX) {
+        count(OP_EP_BB);
         // BC1Encoder.cpp:553-580 (note the operator precedence at :572-573)
         float l[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
         const float bias = 8.0f / 255.0f;
@@ -440,6 +479,7 @@ void find_endpoints(const Enc &e, Result &r, const Block &b, const Metrics &m, i
         r.low = precise_round565(l);
         r.high = precise_round565(h);
     } else if (endpoint_mode == QTX_EP_BOUNDING_BOX_INT) {
+        count(OP_EP_BBINT);
         // BC1Encoder.cpp:581-605
         C4 mn = mk(0, 0, 0), mx = mk(0, 0, 0);
         for (int c = 0; c < 3; c++) {
@@ -458,6 +498,8 @@ void find_endpoints(const Enc &e, Result &r, const Block &b, const Metrics &m, i
         r.low = scale_to565(mn);
         r.high = scale_to565(mx);
     } else if (endpoint_mode == QTX_EP_PCA) {
+        count(OP_EP_PCA);
+        count(OP_PCA_ITER, e.p.power_iterations);
         // BC1Encoder.cpp:606-674
         const float mn[3] = {(float)m.min.v[0], (float)m.min.v[1], (float)m.min.v[2]};
         const float mx[3] = {(float)m.max.v[0], (float)m.max.v[1], (float)m.max.v[2]};
@@ -526,6 +568,10 @@ void find_selectors(const Enc &e, int M, Result &r, const Block &b, int error_mo
         cv[j][0] = c.v[0]; cv[j][1] = c.v[1]; cv[j][2] = c.v[2];
     }
     unsigned total_error = 0;
+    if (error_mode == QTX_ERROR_FULL) count(M == QTX_COLOR_THREE ? OP_SEL_FULL3 : OP_SEL_FULL4);
+    else if (error_mode == QTX_ERROR_CHECK2) count(OP_SEL_CHECK2);
+    else if (error_mode == QTX_ERROR_FASTER) count(OP_SEL_FASTER);
+    else count(OP_SEL_NONE);
     if (error_mode == QTX_ERROR_NONE || error_mode == QTX_ERROR_FASTER) {
         int axis[3] = {cv[3][0] - cv[0][0], cv[3][1] - cv[0][1], cv[3][2] - cv[0][2]};
         int dots[4];
@@ -592,6 +638,7 @@ void find_selectors(const Enc &e, int M, Result &r, const Block &b, int error_mo
 
 // RefineEndpointsLS(result, pixels, metrics): BC1Encoder.cpp:781-825.
 bool refine_endpoints_ls_px(const Enc &e, int M, Result &r, const Block &b, const Metrics &m) {
+    count(OP_LS_PX);
     const int color_count = M & 0x0F;
     const Orders &o = color_count == 4 ? g_o4 : g_o3;
     const int denominator = color_count - 1;
@@ -631,6 +678,7 @@ bool refine_endpoints_ls_px(const Enc &e, int M, Result &r, const Block &b, cons
 // RefineEndpointsLS(result, sums, matrix, hash): BC1Encoder.cpp:827-850.
 void refine_endpoints_ls_sums(int M, Result &r, const float sums[17][3], const float mat[4], const Orders &o,
                               uint16_t hash) {
+    count(OP_LS_SUMS);
     const int color_count = M & 0x0F;
     const int denominator = color_count - 1;
     float q10[3] = {0, 0, 0};
@@ -669,6 +717,7 @@ void refine_block_cf(const Enc &e, int M, Result &r, const Block &b, const Metri
                      unsigned orderings) {
     const int color_count = M & 0x0F;
     const Orders &o = color_count == 4 ? g_o4 : g_o3;
+    count(OP_CF_SETUP);
     const Result orig = r;
     uint8_t h[4] = {0, 0, 0, 0};
     for (int i = 0; i < 16; i++) h[orig.sel[i]]++;
@@ -720,6 +769,7 @@ void endpoint_search(const Enc &e, Result &r, const Block &b) {
         const unsigned vi = i & 15;
         if ((int)(i & 31) == forbidden_direction) continue;
         const int *d = kVoxels[vi];
+        count(OP_SEARCH_ROUND);
         Result t = r;
         C4 &ep = (i & 16) ? t.low : t.high;
         ep.v[0] = (uint8_t)clampi(ep.v[0] + d[0], 0, 31);
@@ -747,6 +797,7 @@ void put_block(uint8_t out[8], uint16_t ep0, uint16_t ep1, const uint8_t rows[4]
 
 // WriteBlockSolid: BC1Encoder.cpp:345-393
 void write_block_solid(const Enc &e, const C4 &color, uint8_t out[8]) {
+    count(OP_SOLID);
     uint8_t mask = 0xAA;
     uint16_t min16, max16;
     if ((color.v[0] | color.v[1] | color.v[2]) == 0) {
@@ -779,6 +830,7 @@ void write_block_solid(const Enc &e, const C4 &color, uint8_t out[8]) {
 
 // WriteBlock: BC1Encoder.cpp:395-441
 void write_block(const Result &r, uint8_t out[8]) {
+    count(OP_WRITE);
     uint16_t ep1 = pack565u(r.low), ep0 = pack565u(r.high);
     uint8_t lut[4];
     if (r.mode & QTX_COLOR_FOUR) {
@@ -880,6 +932,7 @@ void put_bc4(uint8_t out[8], int a0, int a1, const uint8_t sel[16]) {
 }
 
 void encode_bc4_block(const Block &b, int ch, uint8_t out[8]) {
+    count(OP_BC4);
     int mn = 255, mx = 0;
     for (int i = 0; i < 16; i++) {
         mn = std::min(mn, (int)b.px[i].v[ch]);
@@ -1191,6 +1244,12 @@ int qtxo_order_factors(int32_t n, float *out /* count x 4 */) {
     std::memcpy(out, o.factors, sizeof(float) * 4 * o.count);
     return 0;
 }
+
+// Algorithmic op accounting of the calling thread (see OpKind).
+int qtxo_op_kinds(void) { return OP_KINDS; }
+void qtxo_op_reset(void) { std::memset(g_ops, 0, sizeof g_ops); }
+void qtxo_op_counts(uint64_t *out) { std::memcpy(out, g_ops, sizeof g_ops); }
+void qtxo_op_weights(uint32_t *out) { std::memcpy(out, kOpWeight, sizeof kOpWeight); }
 
 int qtxo_midpoints(float *out5 /* 32 */, float *out6 /* 64 */) {
     init_tables();

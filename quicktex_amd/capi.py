@@ -1,0 +1,195 @@
+"""ctypes binding of the C ABI in include/qtx.h (quicktex_amd/libqtx.so).
+
+A thin client for callers that hold their own device memory (torch tensors,
+hipMalloc'ed buffers) and want the ``*_device`` entry points, and for FFI-style
+tests.  The drop-in Python API is the pybind11 module (quicktex_amd.s3tc.*).
+
+Raises ``RuntimeError`` when libqtx.so has not been built: there is no CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libqtx.so')
+
+INTERP_IDEAL, INTERP_IDEAL_ROUND, INTERP_NVIDIA, INTERP_AMD = 0, 1, 2, 3
+COLOR_FOUR, COLOR_THREE, COLOR_THREE_BLACK = 4, 3, 0x13
+FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5 = 1, 3, 4, 5
+
+
+class BC1Params(ctypes.Structure):
+    """qtx_bc1_params (include/qtx.h)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        'interpolator', 'color_mode', 'error_mode', 'endpoint_mode',
+        'two_ls_passes', 'two_ep_passes', 'two_cf_passes', 'exhaustive')] + \
+        [(n, ctypes.c_uint32) for n in ('power_iterations', 'search_rounds', 'orderings4', 'orderings3')]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+
+class QtxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__('qtx status %d: %s' % (status, msg))
+        self.status = status
+
+
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_U8P)
+
+
+def _nb(w, h):
+    return ((w + 3) // 4) * ((h + 3) // 4)
+
+
+class Qtx:
+    """One library handle and one lazily created context on ``device``."""
+
+    def __init__(self, path=LIB_PATH, device=0):
+        if not os.path.exists(path):
+            raise RuntimeError('%s is not built; run `python -m quicktex_amd.build`' % path)
+        self.lib = ctypes.CDLL(path)
+        self.device = device
+        L = self.lib
+        vp, i32, u32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_size_t
+        P = ctypes.POINTER(BC1Params)
+        L.qtx_last_error.restype = ctypes.c_char_p
+        L.qtx_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
+        L.qtx_ctx_destroy.argtypes = [vp]
+        L.qtx_ctx_synchronize.argtypes = [vp]
+        L.qtx_bc1_params_init.argtypes = [P, u32, i32, i32]
+        L.qtx_bc1_params_clamp.argtypes = [P]
+        for n in ('qtx_bc1_encode', 'qtx_bc3_encode'):
+            getattr(L, n).argtypes = [vp, P, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc4_encode.argtypes = [vp, u32, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc5_encode.argtypes = [vp, u32, u32, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc1_decode.argtypes = [vp, i32, i32, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc3_decode.argtypes = [vp, i32, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc4_decode.argtypes = [vp, u32, _U8P, i32, i32, _U8P, sz]
+        L.qtx_bc5_decode.argtypes = [vp, u32, u32, _U8P, i32, i32, _U8P, sz]
+        for n in ('qtx_bc1_encode_device', 'qtx_bc3_encode_device'):
+            getattr(L, n).argtypes = [vp, P, vp, i32, i32, vp, vp]
+        L.qtx_bc4_encode_device.argtypes = [vp, u32, vp, i32, i32, vp, vp]
+        L.qtx_bc5_encode_device.argtypes = [vp, u32, u32, vp, i32, i32, vp, vp]
+        L.qtx_bc1_decode_device.argtypes = [vp, i32, i32, vp, i32, i32, vp, vp]
+        L.qtx_bc3_decode_device.argtypes = [vp, i32, vp, i32, i32, vp, vp]
+        L.qtx_bc4_decode_device.argtypes = [vp, u32, vp, i32, i32, vp, vp]
+        L.qtx_bc5_decode_device.argtypes = [vp, u32, u32, vp, i32, i32, vp, vp]
+        self._ctx = None
+
+    # -- plumbing -------------------------------------------------------------
+    def check(self, st):
+        if st != 0:
+            raise QtxError(st, self.lib.qtx_last_error().decode())
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            c = ctypes.c_void_p()
+            self.check(self.lib.qtx_ctx_create(self.device, ctypes.byref(c)))
+            self._ctx = c
+        return self._ctx
+
+    def close(self):
+        if self._ctx is not None:
+            self.lib.qtx_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def synchronize(self):
+        self.check(self.lib.qtx_ctx_synchronize(self.ctx))
+
+    def params(self, level=5, color_mode=COLOR_FOUR, interpolator=INTERP_IDEAL):
+        p = BC1Params()
+        self.check(self.lib.qtx_bc1_params_init(ctypes.byref(p), level, color_mode, interpolator))
+        return p
+
+    # -- host buffers (numpy) ---------------------------------------------------
+    def bc1_encode(self, rgba, params):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(_nb(w, h) * 8, np.uint8)
+        self.check(self.lib.qtx_bc1_encode(self.ctx, ctypes.byref(params), _ptr(rgba), w, h, _ptr(out), out.size))
+        return out
+
+    def bc3_encode(self, rgba, params):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(_nb(w, h) * 16, np.uint8)
+        self.check(self.lib.qtx_bc3_encode(self.ctx, ctypes.byref(params), _ptr(rgba), w, h, _ptr(out), out.size))
+        return out
+
+    def bc4_encode(self, rgba, channel=3):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(_nb(w, h) * 8, np.uint8)
+        self.check(self.lib.qtx_bc4_encode(self.ctx, channel, _ptr(rgba), w, h, _ptr(out), out.size))
+        return out
+
+    def bc5_encode(self, rgba, chan0=0, chan1=1):
+        h, w = rgba.shape[:2]
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(_nb(w, h) * 16, np.uint8)
+        self.check(self.lib.qtx_bc5_encode(self.ctx, chan0, chan1, _ptr(rgba), w, h, _ptr(out), out.size))
+        return out
+
+    def bc1_decode(self, blocks, w, h, interpolator=INTERP_IDEAL, write_alpha=False):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self.check(self.lib.qtx_bc1_decode(self.ctx, interpolator, int(write_alpha), _ptr(blocks), w, h, _ptr(out),
+                                           out.size))
+        return out
+
+    def bc3_decode(self, blocks, w, h, interpolator=INTERP_IDEAL):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self.check(self.lib.qtx_bc3_decode(self.ctx, interpolator, _ptr(blocks), w, h, _ptr(out), out.size))
+        return out
+
+    def bc4_decode(self, blocks, w, h, channel=3):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self.check(self.lib.qtx_bc4_decode(self.ctx, channel, _ptr(blocks), w, h, _ptr(out), out.size))
+        return out
+
+    def bc5_decode(self, blocks, w, h, chan0=0, chan1=1):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((h, w, 4), np.uint8)
+        self.check(self.lib.qtx_bc5_decode(self.ctx, chan0, chan1, _ptr(blocks), w, h, _ptr(out), out.size))
+        return out
+
+    # -- device buffers (raw pointers, e.g. tensor.data_ptr()) ------------------
+    def bc1_encode_device(self, params, d_rgba, w, h, d_blocks, stream=0):
+        self.check(self.lib.qtx_bc1_encode_device(self.ctx, ctypes.byref(params), d_rgba, w, h, d_blocks, stream))
+
+    def bc3_encode_device(self, params, d_rgba, w, h, d_blocks, stream=0):
+        self.check(self.lib.qtx_bc3_encode_device(self.ctx, ctypes.byref(params), d_rgba, w, h, d_blocks, stream))
+
+    def bc4_encode_device(self, channel, d_rgba, w, h, d_blocks, stream=0):
+        self.check(self.lib.qtx_bc4_encode_device(self.ctx, channel, d_rgba, w, h, d_blocks, stream))
+
+    def bc5_encode_device(self, chan0, chan1, d_rgba, w, h, d_blocks, stream=0):
+        self.check(self.lib.qtx_bc5_encode_device(self.ctx, chan0, chan1, d_rgba, w, h, d_blocks, stream))
+
+    def bc1_decode_device(self, interpolator, write_alpha, d_blocks, w, h, d_rgba, stream=0):
+        self.check(self.lib.qtx_bc1_decode_device(self.ctx, interpolator, int(write_alpha), d_blocks, w, h, d_rgba, stream))
+
+    def bc3_decode_device(self, interpolator, d_blocks, w, h, d_rgba, stream=0):
+        self.check(self.lib.qtx_bc3_decode_device(self.ctx, interpolator, d_blocks, w, h, d_rgba, stream))
+
+    def bc4_decode_device(self, channel, d_blocks, w, h, d_rgba, stream=0):
+        self.check(self.lib.qtx_bc4_decode_device(self.ctx, channel, d_blocks, w, h, d_rgba, stream))
+
+    def bc5_decode_device(self, chan0, chan1, d_blocks, w, h, d_rgba, stream=0):
+        self.check(self.lib.qtx_bc5_decode_device(self.ctx, chan0, chan1, d_blocks, w, h, d_rgba, stream))
+
+
+def declared_symbols(header=os.path.join(os.path.dirname(HERE), 'include', 'qtx.h')):
+    """Every function name declared in include/qtx.h."""
+    import re
+    src = open(header).read()
+    return sorted(set(re.findall(r'\b(qtx_[a-z0-9_]+)\s*\(', src)))

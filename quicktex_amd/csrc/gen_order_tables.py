@@ -11,6 +11,11 @@ data, extracted verbatim (values only) from the reference's table sources:
   * ``OrderTable<3>::BestOrders``  153 x 32 u16          quicktex/s3tc/bc1/OrderTable.cpp:165-321
   * ``OrderTable<4>::BestOrders``  969 x 128 u16         quicktex/s3tc/bc1/OrderTable4.cpp:996-1966
     (the 128-wide branch; ``RGBCX_USE_SMALLER_TABLES`` is undefined in the reference build)
+  * ``OrderTable<3/4>::SingleColorHashes``: the ordering a histogram with all 16
+    selectors in bin i starts from (quicktex/s3tc/bc1/OrderTable.cpp:43-44).  Data,
+    not derived: the 3-colour list is {12, 15, 89} although ordering 15 is the
+    histogram (0, 0, 16) and 89 is (0, 16, 0), i.e. bins 1 and 2 map crosswise,
+    and the encoder's output depends on it.
 
 Everything else the encoder needs from ``OrderTable`` (hash lookup, least
 squares factors) is recomputed from these histograms by the library itself.
@@ -53,11 +58,16 @@ def extract(bc1_dir):
     big = big.split('#else', 1)[1].split('#endif', 1)[0]
     best4 = _ints(big)
 
+    m3 = re.search(r'OrderTable<3>::SingleColorHashes = \{([^}]*)\}', ot)
+    m4 = re.search(r'OrderTable<4>::SingleColorHashes = \{([^}]*)\}', ot)
+    single3 = [int(x) for x in m3.group(1).split(',')]
+    single4 = [int(x) for x in m4.group(1).split(',')]
+    assert len(single3) == 3 and len(single4) == 4
     assert len(orders3) == 153 * 3, len(orders3)
     assert len(orders4) == 969 * 4, len(orders4)
     assert len(best3) == 153 * 32, len(best3)
     assert len(best4) == 969 * 128, len(best4)
-    return orders3, orders4, best3, best4
+    return orders3, orders4, best3, best4, single3, single4
 
 
 def digest(orders3, orders4, best3, best4):
@@ -78,7 +88,7 @@ def _rows(vals, width, per_line):
 
 def main():
     bc1_dir = sys.argv[1] if len(sys.argv) > 1 else '/root/reference/quicktex/s3tc/bc1'
-    orders3, orders4, best3, best4 = extract(bc1_dir)
+    orders3, orders4, best3, best4, single3, single4 = extract(bc1_dir)
     sha = digest(orders3, orders4, best3, best4)
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, 'order_tables.inc'), 'w') as f:
@@ -90,6 +100,9 @@ def main():
         f.write('static const unsigned char qtx_orders4[969 * 4] = {\n%s\n};\n' % _rows(orders4, 4, 48))
         f.write('static const unsigned short qtx_best_orders3[153 * 32] = {\n%s\n};\n' % _rows(best3, 32, 32))
         f.write('static const unsigned short qtx_best_orders4[969 * 128] = {\n%s\n};\n' % _rows(best4, 128, 32))
+        f.write('// ordering index for a histogram with all 16 selectors in bin i (3-colour: bins 1, 2 crosswise)\n')
+        f.write('static const unsigned short qtx_single_color_hashes3[3] = {%s};\n' % ', '.join(map(str, single3)))
+        f.write('static const unsigned short qtx_single_color_hashes4[4] = {%s};\n' % ', '.join(map(str, single4)))
     print('wrote order_tables.inc sha256', sha)
 
 

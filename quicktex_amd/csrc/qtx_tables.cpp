@@ -1,7 +1,8 @@
 // qtx_tables.cpp -- host construction of the codec's constant tables.
 //
 // Built once per process, uploaded once per context, then read-only on the
-// device.  Everything except the trained ordering lists (order_tables.inc) is
+// device.  Everything except the trained ordering data (order_tables.inc: the
+// ordering lists and the single-bin starting orderings) is
 // derived here:
 //   * single-colour match tables, brute force per interpolator
 //     (quicktex/s3tc/bc1/SingleColorTable.h:45-91);
@@ -64,18 +65,14 @@ void build_match(int interp, int bits, uint32_t *out) {
 }
 
 void build_orders(int n, const unsigned char *hist, int count, float (*factors)[4], uint32_t *levels,
-                  uint16_t *hashes, uint16_t *single) {
+                  uint16_t *hashes) {
     // LS normal-equation weights for selector s on a d-step line (OrderTable.cpp:40-41)
     const int d = n - 1;
     const float denom = (float)d;
     for (int i = 0; i < count; i++) {
         const unsigned char *h = hist + i * n;
         bool full = false;
-        for (int s = 0; s < n; s++)
-            if (h[s] == 16) {
-                full = true;
-                single[s] = (uint16_t)i;
-            }
+        for (int s = 0; s < n; s++) full |= h[s] == 16;
         unsigned packed = 0;
         for (int s = 0; s < n - 1; s++) packed |= (unsigned)h[s] << (4 * s);
         if (!full) hashes[packed] = (uint16_t)i;
@@ -124,8 +121,12 @@ const DeviceTables &host_tables() {
             build_match(i, 5, t->match[i][0]);
             build_match(i, 6, t->match[i][1]);
         }
-        build_orders(4, qtx_orders4, kOrders4, t->factors4, t->levels4, t->hashes4, t->single4);
-        build_orders(3, qtx_orders3, kOrders3, t->factors3, t->levels3, t->hashes3, t->single3);
+        build_orders(4, qtx_orders4, kOrders4, t->factors4, t->levels4, t->hashes4);
+        build_orders(3, qtx_orders3, kOrders3, t->factors3, t->levels3, t->hashes3);
+        // starting ordering of a single-bin histogram, by bin (OrderTable.cpp:43-44; the
+        // 3-colour list maps bins 1 and 2 crosswise and is kept as the reference has it)
+        for (int s = 0; s < 4; s++) t->single4[s] = qtx_single_color_hashes4[s];
+        for (int s = 0; s < 3; s++) t->single3[s] = qtx_single_color_hashes3[s];
         std::memcpy(t->best4, qtx_best_orders4, sizeof t->best4);
         std::memcpy(t->best3, qtx_best_orders3, sizeof t->best3);
         build_midpoints(t->mid5, 5);

@@ -13,14 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-class BC1Params(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in (
-        'interpolator', 'color_mode', 'error_mode', 'endpoint_mode',
-        'two_ls_passes', 'two_ep_passes', 'two_cf_passes', 'exhaustive')] + \
-        [(n, ctypes.c_uint32) for n in ('power_iterations', 'search_rounds', 'orderings4', 'orderings3')]
-
-    def as_tuple(self):
-        return tuple(getattr(self, n) for n, _ in self._fields_)
+from quicktex_amd.capi import BC1Params  # noqa: E402  (mirrors include/qtx.h)
 
 
 _U8P = ctypes.POINTER(ctypes.c_uint8)
@@ -133,3 +126,21 @@ def reference():
     if _ref is None and os.path.exists(reference_path()):
         _ref = _Codec(reference_path(), 'qtxr_')
     return _ref
+
+
+def op_count(fn):
+    """Run fn() on the oracle single-threaded and return (counts, weights) of
+    the algorithmic-op accounting (oracle/qtx_oracle.cpp, OpKind)."""
+    o = oracle()
+    n = o.lib.qtxo_op_kinds()
+    o.lib.qtxo_set_threads(1)
+    o.lib.qtxo_op_reset()
+    try:
+        fn(o)
+    finally:
+        o.lib.qtxo_set_threads(0)
+    counts = (ctypes.c_uint64 * n)()
+    weights = (ctypes.c_uint32 * n)()
+    o.lib.qtxo_op_counts(counts)
+    o.lib.qtxo_op_weights(weights)
+    return list(counts), list(weights)

@@ -1,7 +1,14 @@
-"""GPU parity: the HIP kernels, called through the C ABI, against the oracle.
+"""GPU parity: the HIP kernels, called through the C ABI, against the CPU checker.
 
-Bit-exact for every byte: BC1 at every level (0-19), colour mode and
-interpolator; BC3 with varying alpha; BC4/BC5; every decoder, on ragged sizes.
+The checker is the reference's own C++ (oracle/_ref) where it was built, else
+the oracle restatement.  Bit-exact for every byte: BC1 at every level (0-19),
+colour mode and interpolator on a mixture image and a stress image of
+two/three-colour, greyscale and near-black blocks; BC3 with varying alpha;
+BC4/BC5; every decoder, on ragged sizes.
+
+Level 1 (EndpointMode.LeastSquares) with the Nvidia interpolator is compared
+with the oracle only: the reference reads an uninitialised accumulator there
+(quicktex/s3tc/bc1/BC1Encoder.cpp:510) and its output depends on stack garbage.
 """
 import numpy as np
 import pytest
@@ -22,34 +29,60 @@ def _mismatch(a, b, bs):
     return bad
 
 
+def stress_rgba(w, h, seed):
+    """Blocks of 2-3 distinct colours: greyscale, near-black, random."""
+    rng = np.random.default_rng(seed)
+    img = np.zeros((h, w, 4), np.uint8)
+    img[..., 3] = 255
+    for by in range(h // 4):
+        for bx in range(w // 4):
+            kind = rng.integers(0, 5)
+            cols = rng.integers(0, 256, (3, 3))
+            if kind == 0:
+                cols[:, :] = cols[:, :1]
+            if kind == 1:
+                cols[1] = rng.integers(0, 8, 3)
+            pick = rng.integers(0, 3 if kind > 2 else 2, 16)
+            img[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4, :3] = cols[pick].reshape(4, 4, 3)
+    return img
+
+
 @pytest.fixture(scope='module')
-def img64():
-    return synth.mixture_rgba(64, 64, seed=0x5EED0000 + 7, near_black=0.1)
+def images():
+    return [synth.mixture_rgba(512, 32, seed=0x5EED0000 + 7, near_black=0.15), stress_rgba(256, 32, 11)]
+
+
+@pytest.fixture(scope='module')
+def checker():
+    from tests._cpu_libs import oracle, reference
+    return reference() or oracle()
 
 
 @pytest.mark.parametrize('interp', INTERPS)
 @pytest.mark.parametrize('mode', MODES)
 @pytest.mark.parametrize('level', LEVELS)
-def test_bc1_encode_levels(qtx, oracle, img64, level, mode, interp):
+def test_bc1_encode_levels(qtx, oracle, checker, images, level, mode, interp):
     p = qtx.params(level, mode, interp)
     assert p.as_tuple() == oracle.params(level, mode, interp).as_tuple()
-    got = qtx.bc1_encode(img64, p)
-    want = oracle.bc1_encode(img64, p)
-    bad = _mismatch(got, want, 8)
-    assert len(bad) == 0, 'blocks %s: gpu %s oracle %s' % (
-        bad[:4].tolist(), got.reshape(-1, 8)[bad[0]].tobytes().hex(), want.reshape(-1, 8)[bad[0]].tobytes().hex())
+    cpu = oracle if (level == 1 and interp == 2) else checker
+    for k, img in enumerate(images):
+        got = qtx.bc1_encode(img, p)
+        want = cpu.bc1_encode(img, p)
+        bad = _mismatch(got, want, 8)
+        assert len(bad) == 0, 'image %d blocks %s: gpu %s cpu %s' % (
+            k, bad[:4].tolist(), got.reshape(-1, 8)[bad[0]].tobytes().hex(), want.reshape(-1, 8)[bad[0]].tobytes().hex())
 
 
 @pytest.mark.parametrize('interp', INTERPS)
 @pytest.mark.parametrize('level', [0, 2, 5, 10, 13, 18])
-def test_bc3_encode_alpha(qtx, oracle, level, interp):
+def test_bc3_encode_alpha(qtx, checker, level, interp):
     img = synth.mixture_rgba(48, 40, seed=99)
     rng = np.random.default_rng(level * 4 + interp)
     img[..., 3] = rng.integers(0, 256, img.shape[:2], dtype=np.uint8)
     img[:8, :8, 3] = 255  # some opaque blocks
     p = qtx.params(level, 4, interp)
     got = qtx.bc3_encode(img, p)
-    want = oracle.bc3_encode(img, p)
+    want = checker.bc3_encode(img, p)
     bad = _mismatch(got, want, 16)
     assert len(bad) == 0, 'blocks %s' % bad[:8].tolist()
 
