@@ -1,0 +1,119 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ (run where the reference tree is).
+
+1. reference_tests.json -- the known-answer data the reference's own tests hold:
+   * the 4x4 test images tests/images/bc1/{greyscale,3color,3color_black}_unpacked.png
+     and tests/images/bc4/{6value,8value}.png as RGBA texel lists;
+   * their expected blocks, as the reference tests state them
+     (tests/images/__init__.py:18-20, 31-32; test_bc1.py:9-12; test_bc4.py:9-11, 58-70).
+2. ref_vectors.npz -- outputs of the reference C++ compiled from its own sources
+   (oracle/_ref/libqtxref.so) on a fixed synthetic input: BC1 at every level,
+   colour mode and interpolator, BC3, BC4, BC5 and every decoder.
+
+Both files are data only (inputs and expected outputs).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+REF_TESTS = '/root/reference/tests/images'
+
+
+def reference_tests():
+    from PIL import Image
+
+    def texels(path):
+        img = Image.open(path).convert('RGBA')
+        assert img.size == (4, 4)
+        return list(img.tobytes('raw', 'RGBA'))
+
+    data = {
+        'bc1': {
+            # tests/images/__init__.py:18-20
+            'greyscale': {'texels': texels(os.path.join(REF_TESTS, 'bc1', 'greyscale_unpacked.png')),
+                          'block': 'ffff494a78787878'},
+            'three_color': {'texels': texels(os.path.join(REF_TESTS, 'bc1', '3color_unpacked.png')),
+                            'block': 'e00700f829292929'},
+            'three_color_black': {'texels': texels(os.path.join(REF_TESTS, 'bc1', '3color_black_unpacked.png')),
+                                  'block': 'e00700f827272727'},
+        },
+        'bc1_block': {  # test_bc1.py:9-12
+            'in_endpoints': [[253, 254, 255], [65, 70, 67]],
+            'out_endpoints': [[255, 255, 255, 255], [66, 69, 66, 255]],
+            'selectors': [[0, 2, 3, 1]] * 4,
+            'bytes': 'ffff284278787878',
+        },
+        'bc4': {
+            # tests/images/__init__.py:31-32 (BC4Block(8, 248, ...) and BC4Block(240, 16, ...))
+            'six_value': {'texels': texels(os.path.join(REF_TESTS, 'bc4', '6value.png')),
+                          'endpoints': [8, 248], 'selectors': [[0, 1, 2, 3]] * 2 + [[4, 5, 6, 7]] * 2},
+            'eight_value': {'texels': texels(os.path.join(REF_TESTS, 'bc4', '8value.png')),
+                            'endpoints': [240, 16], 'selectors': [[0, 1, 2, 3]] * 2 + [[4, 5, 6, 7]] * 2},
+        },
+        'bc4_block': {  # test_bc4.py:9-11, 58-70
+            'bytes': 'f010888668accffa', 'endpoints': [240, 16],
+            'selectors': [[0, 1, 2, 3]] * 2 + [[4, 5, 6, 7]] * 2,
+            'values_6': [8, 248, 56, 104, 152, 200, 0, 255],
+            'values_8': [240, 16, 208, 176, 144, 112, 80, 48],
+        },
+    }
+    with open(os.path.join(HERE, 'reference_tests.json'), 'w') as f:
+        json.dump(data, f, indent=1)
+
+
+def ref_vectors():
+    from quicktex_amd import synth
+    from tests._cpu_libs import reference
+    from tests.test_gpu_parity import stress_rgba
+    R = reference()
+    assert R is not None, 'build oracle/_ref first (make -C oracle ref)'
+    mix = synth.mixture_rgba(64, 16, seed=0x601DE0, near_black=0.15)
+    stress = stress_rgba(64, 16, 0x601DE1)
+    img = np.concatenate([mix, stress], axis=0)  # 64 x 32 -> 128 blocks
+    out = {'bc1_input': img}
+    for interp in range(4):
+        for mode in (4, 3, 0x13):
+            for level in range(20):
+                if level == 1 and interp == 2:
+                    continue  # reads uninitialised memory in the reference
+                p = R.params(level, mode, interp)
+                out['bc1_L%d_M%d_I%d' % (level, mode, interp)] = R.bc1_encode(img, p)
+    rng = np.random.default_rng(0x601DE2)
+    a = img.copy()
+    a[..., 3] = rng.integers(0, 256, a.shape[:2], dtype=np.uint8)
+    out['bc3_input'] = a
+    for interp in range(4):
+        for level in (0, 5, 10, 18):
+            out['bc3_L%d_I%d' % (level, interp)] = R.bc3_encode(a, R.params(level, 4, interp))
+    sm = synth.smooth_channels(37, 23, seed=0x601DE3)
+    sm[..., 2:] = rng.integers(0, 256, (23, 37, 2), dtype=np.uint8)
+    out['bc45_input'] = sm
+    for ch in range(4):
+        out['bc4_C%d' % ch] = R.bc4_encode(sm, ch)
+    out['bc5_C01'] = R.bc5_encode(sm, 0, 1)
+    out['bc5_C32'] = R.bc5_encode(sm, 3, 2)
+    w, h = 37, 23
+    nb = ((w + 3) // 4) * ((h + 3) // 4)
+    b8 = rng.integers(0, 256, nb * 8, dtype=np.uint8)
+    b16 = rng.integers(0, 256, nb * 16, dtype=np.uint8)
+    out['dec_blocks8'] = b8
+    out['dec_blocks16'] = b16
+    for interp in range(4):
+        for wa in (0, 1):
+            out['dec_bc1_I%d_A%d' % (interp, wa)] = R.bc1_decode(b8, w, h, interp, bool(wa))
+        out['dec_bc3_I%d' % interp] = R.bc3_decode(b16, w, h, interp)
+    for ch in range(4):
+        out['dec_bc4_C%d' % ch] = R.bc4_decode(b8, w, h, ch)
+    out['dec_bc5_C01'] = R.bc5_decode(b16, w, h, 0, 1)
+    np.savez_compressed(os.path.join(HERE, 'ref_vectors.npz'), **out)
+
+
+if __name__ == '__main__':
+    reference_tests()
+    ref_vectors()
+    print('wrote', os.listdir(HERE))
