@@ -13,6 +13,8 @@
 //   texels, so no two lanes write one pixel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bc1_device.h"
 #include "kernels.h"
 
@@ -70,36 +72,42 @@ __device__ __forceinline__ uint32_t bc4_value(uint32_t a0, uint32_t a1, uint32_t
 // ---------------------------------------------------------------------------
 template <int INTERP, bool BC3>
 __global__ __launch_bounds__(256, 4) void k_bc1_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
-                                                    uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
-                                                    BC1Settings s) {
-    const int gid = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
-    const bool live = gid < nblocks;
-    const int bx = gid % bw, by = gid / bw;
-    // lane i of the group loads texel i into the group's LDS slots
-    if (live) qtx_lds[threadIdx.x] = load_texel(rgba, w, h, bx, by, glane());
-    if (threadIdx.x == 0) *reinterpret_cast<BC1Settings *>(qtx_lds + kLdsTexWords) = s;
+                                                       uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                       BC1Settings s) {
+    load_lds_tables<INTERP>(T, s);
     __syncthreads();
-    if (!live) return;
-    int pp = 0;
-    uint32_t ppa = 0;
-    for (int q = 0; q < 4; q++) {
-        const uint4 t = tex4(q);
-        pp += (int)(dot4(t.x & 0xFFFFFFu, t.x & 0xFFFFFFu) + dot4(t.y & 0xFFFFFFu, t.y & 0xFFFFFFu) +
-                    dot4(t.z & 0xFFFFFFu, t.z & 0xFFFFFFu) + dot4(t.w & 0xFFFFFFu, t.w & 0xFFFFFFu));
-        ppa += dot4(t.x, t.x) + dot4(t.y, t.y) + dot4(t.z, t.z) + dot4(t.w, t.w);
-    }
-    const Ctx cx{T, pp, ppa};
-    const uint2 colour = encode_bc1<INTERP>(cx);
-    if (glane() == 0) {
+    // workgroups stride over chunks of 16 blocks; each group of 16 lanes owns one
+    // block of the chunk and only touches its own 16 LDS texel words
+    const int nchunks = (nblocks + 15) / 16;
+    const int group = (int)(threadIdx.x / G);
+    for (int chunk = (int)blockIdx.x; chunk < nchunks; chunk += (int)gridDim.x) {
+        const int gid = chunk * 16 + group;
+        if (gid >= nblocks) break;
+        const int bx = gid % bw, by = gid / bw;
+        const uint32_t px = load_texel(rgba, w, h, bx, by, glane());
+        L().tex[threadIdx.x] = px;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t rgb0 = px & 0x00FFFFFFu;
+        const int pp = (int)gsum(dot4(rgb0, rgb0));
+        const uint32_t ppa = gsum(dot4(px, px));
+        const Ctx cx{T, px, pp, ppa};
+        const uint2 colour = encode_bc1<INTERP>(cx);
+        uint2 alpha = make_uint2(0, 0);
         if (BC3) {
             uint32_t a[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) a[i] = tex(i) >> 24;
-            const uint2 alpha = bc4_encode_values(a);
-            reinterpret_cast<uint4 *>(out)[gid] = make_uint4(alpha.x, alpha.y, colour.x, colour.y);
-        } else {
-            reinterpret_cast<uint2 *>(out)[gid] = colour;
+            alpha = bc4_encode_values(a);
         }
+        if (glane() == 0) {
+            if (BC3) reinterpret_cast<uint4 *>(out)[gid] = make_uint4(alpha.x, alpha.y, colour.x, colour.y);
+            else reinterpret_cast<uint2 *>(out)[gid] = colour;
+        }
+        // the next chunk's texels overwrite this group's LDS words: keep the reads above them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -254,7 +262,9 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ bloc
 // Launchers
 namespace {
 constexpr int kThreads = 256;
-constexpr size_t kBc1Lds = dev::kLdsTexWords * 4 + sizeof(BC1Settings);
+constexpr size_t kBc1Lds = sizeof(dev::Lds);
+// persistent workgroups for the BC1/BC3 encoder: 256 CUs x 4 resident workgroups
+constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
 }  // namespace
 
@@ -262,7 +272,7 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
                              const BC1Settings &s, int interp, bool bc3, hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
-    const dim3 grid(grid_for((long long)nblocks * dev::G, kThreads)), block(kThreads);
+    const dim3 grid((unsigned)std::min<long long>(((long long)nblocks + 15) / 16, kBc1MaxGrid)), block(kThreads);
 #define QTX_BC1_CASE(I)                                                                                             \
     case I:                                                                                                         \
         if (bc3)                                                                                                    \
