@@ -47,6 +47,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-rows', type=int, default=512, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only run 3 device-to-device copies of MIB MiB (calibrates the HBM byte counters) and exit')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: the codec has no multi-GPU path (textures shard with no '
@@ -54,22 +56,31 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
-    from quicktex_amd import synth
+    from quicktex_amd import hip, synth
     from quicktex_amd.capi import COLOR_THREE_BLACK, Qtx
+
+    q = Qtx()
+    q.ctx  # device set-up through the library's HIP runtime
+    if a.copy_probe:
+        n = a.copy_probe << 20
+        src = hip.Buffer(n)
+        src.zero(7)
+        dst = hip.Buffer(n)
+        for _ in range(3):
+            dst.copy_from(src)
+        print(json.dumps({'copy_probe_bytes': n, 'copies': 3}), flush=True)
+        return
 
     log('generating %dx%d texture' % (W, H))
     img = synth.mixture_rgba(W, H, seed=0x5EED0002, near_black=0.10)
-    dev = torch.device('cuda:0')
-    d_in = torch.from_numpy(img).to(dev).contiguous()
+    d_in = hip.Buffer.from_array(img)
     nb = (W // 4) * (H // 4)
-    d_out = torch.empty(4 * nb * 8, dtype=torch.uint8, device=dev)
-    q = Qtx()
+    d_out = hip.Buffer(4 * nb * 8)
     params = [q.params(18, COLOR_THREE_BLACK, i) for i in range(4)]
-    stream = torch.cuda.current_stream()
-    sh = stream.cuda_stream
-    in_ptr, out_ptr = d_in.data_ptr(), d_out.data_ptr()
+    stream = hip.Stream()
+    sh = stream.handle
+    in_ptr, out_ptr = d_in.ptr, d_out.ptr
 
     def launch(i):
         q.bc1_encode_device(params[i], in_ptr, W, H, out_ptr + i * nb * 8, sh)
@@ -78,11 +89,10 @@ def main():
     for _ in range(a.warmup):
         for i in range(4):
             launch(i)
-    torch.cuda.synchronize()
+    stream.synchronize()
 
     # per-launch kernel durations, HIP events on the launch stream
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(4 * a.steps)]
-    torch.cuda.synchronize()
+    ev = [[hip.Event() for _ in range(2)] for _ in range(4 * a.steps)]
     t0 = time.perf_counter()
     for s in range(a.steps):
         for i in range(4):
@@ -90,15 +100,15 @@ def main():
             e[0].record(stream)
             launch(i)
             e[1].record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     t1 = time.perf_counter()
-    dur = np.array([e[0].elapsed_time(e[1]) for e in ev]).reshape(a.steps, 4)  # ms
+    dur = np.array([e[0].elapsed_ms(e[1]) for e in ev]).reshape(a.steps, 4)  # ms
     step_ms = (t1 - t0) * 1000.0 / a.steps
     px_step = 4 * W * H
     value = px_step / (step_ms / 1000.0) / 1e6
     per_interp = {INTERPS[i]: round(W * H / (dur[:, i].mean() / 1000.0) / 1e6, 1) for i in range(4)}
     log('step %.2f ms, %.1f Mpx/s' % (step_ms, value))
-    gpu_out = d_out.cpu().numpy().reshape(4, nb, 8)
+    gpu_out = d_out.download().reshape(4, nb, 8)
 
     # algorithmic ops: oracle accounting on a 16-row sample, scaled per launch
     from tests._cpu_libs import op_count

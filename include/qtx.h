@@ -150,6 +150,61 @@ qtx_status qtx_bc4_decode(qtx_ctx *ctx, uint32_t channel, const uint8_t *blocks,
 qtx_status qtx_bc5_decode(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, const uint8_t *blocks, int32_t width,
                           int32_t height, uint8_t *rgba_out, size_t out_bytes);
 
+/* ---------------------------------------------------------------------- */
+/* Batched form: many textures (of any sizes, formats and encoder settings)
+ * in one launch.  Replaces the per-texture loops of the reference's callers:
+ * the mip chain of quicktex.dds.encode (quicktex/dds.py:282-287, one
+ * encoder.encode per mip) and DDSFile.decode_all (quicktex/dds.py:209-216,
+ * one decoder.decode per mip).  Each job produces the same bytes as the
+ * matching plain entry point above. */
+enum { QTX_OP_ENCODE = 0, QTX_OP_DECODE = 1 };
+
+typedef struct qtx_job {
+    int32_t op;            /* QTX_OP_ENCODE or QTX_OP_DECODE                         */
+    int32_t format;        /* QTX_FMT_BC1, _BC3, _BC4 or _BC5                        */
+    int32_t width, height; /* texture size in pixels                                 */
+    const void *src;       /* device memory: RGBA8 texture (encode) or blocks (decode) */
+    void *dst;             /* device memory: blocks (encode) or RGBA8 texture (decode) */
+    qtx_bc1_params bc1;    /* BC1/BC3 encode: encoder state; BC1/BC3 decode: only
+                              bc1.interpolator is read                              */
+    int32_t chan0, chan1;  /* BC4: chan0; BC5: chan0 and chan1                       */
+    int32_t write_alpha;   /* BC1 decode                                             */
+    int32_t reserved;      /* 0                                                      */
+} qtx_job;
+
+/* Launch shape of a batch, chosen by qtx_batch_autotune or set by hand. */
+typedef struct qtx_batch_tuning {
+    int32_t workgroups_per_cu; /* resident BC1/BC3 workgroups per CU, 1..4 (256 lanes each) */
+    uint32_t unit_kops;        /* target work per queue unit, in thousands of ops, 16..16384 */
+} qtx_batch_tuning;
+
+typedef struct qtx_batch qtx_batch;
+/* Validates the jobs (EINVAL/ERANGE as the plain entry points would), orders
+ * them by estimated cost, uploads the job tables and captures the launch as a
+ * HIP graph (one persistent BC1/BC3 kernel per interpolator present, on
+ * parallel branches, and one kernel for BC4/BC5 encodes and all decodes).
+ * Pointers in the jobs are captured: the batch encodes whatever those buffers
+ * hold at each launch.  Destination buffers must not overlap. */
+qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx_batch **out);
+/* Enqueues one run of the batch on stream (NULL = null stream).  Runs of one
+ * batch are serialised among themselves, whatever streams they are given. */
+qtx_status qtx_batch_launch(qtx_batch *batch, void *stream);
+/* Times `repeats` launches of each candidate shape on stream and keeps the
+ * fastest (the runs write the outputs; results do not depend on the shape).
+ * Writes the choice and its mean time per launch (ms) if the pointers are
+ * non-NULL.  Synchronises the stream. */
+qtx_status qtx_batch_autotune(qtx_batch *batch, void *stream, int32_t repeats, qtx_batch_tuning *best,
+                              float *best_ms);
+qtx_status qtx_batch_set_tuning(qtx_batch *batch, const qtx_batch_tuning *tuning);
+qtx_status qtx_batch_get_tuning(const qtx_batch *batch, qtx_batch_tuning *tuning);
+qtx_status qtx_batch_destroy(qtx_batch *batch);
+/* Convenience: create, launch on stream, wait, destroy. */
+qtx_status qtx_batch_run(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, void *stream);
+/* Host-memory form: src/dst of the jobs are host pointers; inputs are staged
+ * into one device arena, the batch runs once, outputs are copied back.
+ * dst_bytes[i] is the capacity of jobs[i].dst. */
+qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *dst_bytes, size_t njobs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,35 @@ class BC1Params(ctypes.Structure):
         return tuple(getattr(self, n) for n, _ in self._fields_)
 
 
+OP_ENCODE, OP_DECODE = 0, 1
+
+
+class Job(ctypes.Structure):
+    """qtx_job (include/qtx.h): one texture of a batch."""
+    _fields_ = [('op', ctypes.c_int32), ('format', ctypes.c_int32), ('width', ctypes.c_int32),
+                ('height', ctypes.c_int32), ('src', ctypes.c_void_p), ('dst', ctypes.c_void_p),
+                ('bc1', BC1Params), ('chan0', ctypes.c_int32), ('chan1', ctypes.c_int32),
+                ('write_alpha', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
+class BatchTuning(ctypes.Structure):
+    """qtx_batch_tuning (include/qtx.h)."""
+    _fields_ = [('workgroups_per_cu', ctypes.c_int32), ('unit_kops', ctypes.c_uint32)]
+
+
+def job(op, fmt, w, h, src, dst, params=None, chan0=0, chan1=1, write_alpha=False, interpolator=None):
+    """A Job; params is a BC1Params for BC1/BC3 encodes, interpolator for BC1/BC3 decodes."""
+    j = Job()
+    j.op, j.format, j.width, j.height = op, fmt, w, h
+    j.src, j.dst = src, dst
+    if params is not None:
+        j.bc1 = params
+    if interpolator is not None:
+        j.bc1.interpolator = interpolator
+    j.chan0, j.chan1, j.write_alpha = chan0, chan1, int(write_alpha)
+    return j
+
+
 class QtxError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__('qtx status %d: %s' % (status, msg))
@@ -80,6 +109,16 @@ class Qtx:
         L.qtx_bc3_decode_device.argtypes = [vp, i32, vp, i32, i32, vp, vp]
         L.qtx_bc4_decode_device.argtypes = [vp, u32, vp, i32, i32, vp, vp]
         L.qtx_bc5_decode_device.argtypes = [vp, u32, u32, vp, i32, i32, vp, vp]
+        JP = ctypes.POINTER(Job)
+        TP = ctypes.POINTER(BatchTuning)
+        L.qtx_batch_create.argtypes = [vp, JP, sz, ctypes.POINTER(vp)]
+        L.qtx_batch_launch.argtypes = [vp, vp]
+        L.qtx_batch_autotune.argtypes = [vp, vp, i32, TP, ctypes.POINTER(ctypes.c_float)]
+        L.qtx_batch_set_tuning.argtypes = [vp, TP]
+        L.qtx_batch_get_tuning.argtypes = [vp, TP]
+        L.qtx_batch_destroy.argtypes = [vp]
+        L.qtx_batch_run.argtypes = [vp, JP, sz, vp]
+        L.qtx_batch_run_host.argtypes = [vp, JP, ctypes.POINTER(sz), sz]
         self._ctx = None
 
     # -- plumbing -------------------------------------------------------------
@@ -186,6 +225,60 @@ class Qtx:
 
     def bc5_decode_device(self, chan0, chan1, d_blocks, w, h, d_rgba, stream=0):
         self.check(self.lib.qtx_bc5_decode_device(self.ctx, chan0, chan1, d_blocks, w, h, d_rgba, stream))
+
+
+class Batch:
+    """A qtx_batch over device buffers: create once, launch many times."""
+
+    def __init__(self, q, jobs):
+        self.q = q
+        arr = (Job * len(jobs))(*jobs)
+        h = ctypes.c_void_p()
+        q.check(q.lib.qtx_batch_create(q.ctx, arr, len(jobs), ctypes.byref(h)))
+        self.h = h
+
+    def launch(self, stream=0):
+        self.q.check(self.q.lib.qtx_batch_launch(self.h, stream))
+
+    def autotune(self, stream=0, repeats=2):
+        t = BatchTuning()
+        ms = ctypes.c_float()
+        self.q.check(self.q.lib.qtx_batch_autotune(self.h, stream, repeats, ctypes.byref(t), ctypes.byref(ms)))
+        return (t.workgroups_per_cu, t.unit_kops), ms.value
+
+    @property
+    def tuning(self):
+        t = BatchTuning()
+        self.q.check(self.q.lib.qtx_batch_get_tuning(self.h, ctypes.byref(t)))
+        return (t.workgroups_per_cu, t.unit_kops)
+
+    @tuning.setter
+    def tuning(self, v):
+        t = BatchTuning(*v)
+        self.q.check(self.q.lib.qtx_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def close(self):
+        if self.h is not None:
+            self.q.lib.qtx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_host_batch(q, jobs):
+    """qtx_batch_run_host over numpy buffers: jobs is a list of (Job, src_array, dst_array)."""
+    arr = (Job * len(jobs))()
+    caps = (ctypes.c_size_t * len(jobs))()
+    for i, (j, src, dst) in enumerate(jobs):
+        j.src = src.ctypes.data
+        j.dst = dst.ctypes.data
+        arr[i] = j
+        caps[i] = dst.nbytes
+    q.check(q.lib.qtx_batch_run_host(q.ctx, arr, caps, len(jobs)))
 
 
 def declared_symbols(header=os.path.join(os.path.dirname(HERE), 'include', 'qtx.h')):

@@ -1,0 +1,437 @@
+// batch.cpp -- the batched form of the codec (include/qtx.h, qtx_batch_*).
+//
+// A batch is a fixed list of jobs (one texture each, any size, format and
+// encoder settings) launched as one HIP graph:
+//   * BC1/BC3 encodes run in one persistent kernel per interpolator present
+//     (k_color_batch); its waves take work units -- runs of consecutive blocks
+//     of one job -- from a queue in device memory, so textures of any size share
+//     the whole GPU and no block waits for a launch of its own;
+//   * BC4/BC5 encodes and every decode run in one kernel with a workgroup per
+//     256 blocks (k_light_batch).
+// The kernels are independent branches of the graph.  Jobs are ordered by
+// estimated cost, largest first, so the queue ends on small units.  The launch
+// shape (resident workgroups per CU, work per unit) is a tuning that
+// qtx_batch_autotune picks by timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "kernels.h"
+#include "qtx_internal.h"
+#include "runtime.h"
+
+using namespace qtx;
+using namespace qtx::rt;
+
+namespace {
+
+constexpr int kMaxColourWgPerCu = 4;  // __launch_bounds__(256, 4) of the colour kernel
+constexpr uint32_t kMaxRounds = 1024;  // a colour unit holds at most 4 * kMaxRounds blocks
+
+struct Prepared {
+    BatchJob job;
+    double block_ops;  // estimated ALU ops per block
+    int group;         // interpolator 0..3 for colour jobs, 4 for light jobs
+    size_t src_bytes, dst_bytes;
+};
+
+// Estimated ALU ops per BC1 block for the encoder settings: a fit of the
+// oracle's op accounting on the block mixture (levels 0-19): about 2k ops of
+// metrics, endpoints and selectors, 500 per cluster-fit ordering and pass,
+// and 780 per endpoint-search round, of which about 37 run before the search
+// stops for lack of improvement.
+double bc1_block_ops(const BC1Settings &s) {
+    const double o4 = s.exhaustive ? kOrders4 : s.orderings4;
+    const double o3 = s.exhaustive ? kOrders3 : s.orderings3;
+    const bool three = (s.color_mode & QTX_COLOR_THREE) != 0;
+    double ops = 2000.0 + s.cf_passes * (o4 * 500.0 + (three ? o3 * 500.0 + 1000.0 : 0.0));
+    ops += 780.0 * std::min<double>(s.search_rounds, 37.0);
+    return ops;
+}
+
+qtx_status job_fail(size_t i) { return fail(QTX_EINVAL, "job %zu: %s", i, last_error()); }
+
+qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
+    std::memset(&p, 0, sizeof p);
+    qtx_status st = check_dims(j.width, j.height);
+    if (st != QTX_OK) return fail(st, "job %zu: %s", i, last_error());
+    if (!j.src || !j.dst) return fail(QTX_EINVAL, "job %zu: null buffer", i);
+    BatchJob &b = p.job;
+    b.src = static_cast<const uint8_t *>(j.src);
+    b.dst = static_cast<uint8_t *>(j.dst);
+    b.w = j.width;
+    b.h = j.height;
+    b.bw = (j.width + 3) / 4;
+    b.bh = (j.height + 3) / 4;
+    b.nblocks = (uint32_t)b.bw * (uint32_t)b.bh;
+    const int bpb = (j.format == QTX_FMT_BC1 || j.format == QTX_FMT_BC4) ? 8 : 16;
+    if (j.format != QTX_FMT_BC1 && j.format != QTX_FMT_BC3 && j.format != QTX_FMT_BC4 && j.format != QTX_FMT_BC5)
+        return fail(QTX_EINVAL, "job %zu: unknown format %d", i, j.format);
+    if (j.op == QTX_OP_ENCODE) {
+        p.src_bytes = raw_bytes(j.width, j.height);
+        p.dst_bytes = block_bytes(j.width, j.height, bpb);
+        if (j.format == QTX_FMT_BC1 || j.format == QTX_FMT_BC3) {
+            int interp = 0;
+            if (to_settings(&j.bc1, b.s, interp) != QTX_OK) return job_fail(i);
+            if (j.format == QTX_FMT_BC3 && b.s.color_mode != QTX_COLOR_FOUR)
+                return fail(QTX_EINVAL, "job %zu: BC3 colour blocks are always FourColor", i);
+            b.kind = j.format == QTX_FMT_BC1 ? BJ_ENC_BC1 : BJ_ENC_BC3;
+            b.interp = interp;
+            p.group = interp;
+            p.block_ops = bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
+        } else {
+            if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
+                return fail(QTX_EINVAL, "job %zu: Channel out of range", i);
+            b.kind = j.format == QTX_FMT_BC4 ? BJ_ENC_BC4 : BJ_ENC_BC5;
+            b.p0 = j.chan0;
+            b.p1 = j.format == QTX_FMT_BC5 ? j.chan1 : 0;
+            p.group = 4;
+            p.block_ops = j.format == QTX_FMT_BC4 ? 300.0 : 600.0;
+        }
+    } else if (j.op == QTX_OP_DECODE) {
+        p.src_bytes = block_bytes(j.width, j.height, bpb);
+        p.dst_bytes = raw_bytes(j.width, j.height);
+        p.group = 4;
+        p.block_ops = 200.0;
+        switch (j.format) {
+            case QTX_FMT_BC1:
+            case QTX_FMT_BC3:
+                if (j.bc1.interpolator < 0 || j.bc1.interpolator > 3)
+                    return fail(QTX_EINVAL, "job %zu: Invalid interpolator type", i);
+                b.kind = j.format == QTX_FMT_BC1 ? BJ_DEC_BC1 : BJ_DEC_BC3;
+                b.interp = j.bc1.interpolator;
+                b.p0 = j.format == QTX_FMT_BC1 ? (j.write_alpha != 0) : 0;
+                break;
+            default:
+                if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
+                    return fail(QTX_EINVAL, "job %zu: Channel out of range", i);
+                b.kind = j.format == QTX_FMT_BC4 ? BJ_DEC_BC4 : BJ_DEC_BC5;
+                b.p0 = j.chan0;
+                b.p1 = j.format == QTX_FMT_BC5 ? j.chan1 : 0;
+                break;
+        }
+    } else {
+        return fail(QTX_EINVAL, "job %zu: unknown op %d", i, j.op);
+    }
+    return QTX_OK;
+}
+
+}  // namespace
+
+struct qtx_batch {
+    struct Group {
+        int interp = -1;           // 0..3 colour, -1 light
+        std::vector<int> order;    // job indices, descending total cost
+        size_t first = 0;          // offset into the device job array
+        uint32_t nunits = 0;
+        unsigned grid = 0;
+    };
+    qtx_ctx *ctx = nullptr;
+    std::vector<Prepared> jobs;
+    std::vector<Group> groups;  // colour groups (by interpolator), then the light group
+    BatchJob *d_jobs = nullptr;
+    uint32_t *d_queues = nullptr;  // two words per group
+    qtx_batch_tuning tuning{kMaxColourWgPerCu, 256};
+    hipStream_t cap = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr, done = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void drop_graph(qtx_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Unit layout for the tuning, upload of the job array, capture of the graph.
+qtx_status rebuild(qtx_batch *b) {
+    const qtx_batch_tuning &t = b->tuning;
+    std::vector<BatchJob> host;
+    host.reserve(b->jobs.size());
+    for (auto &g : b->groups) {
+        g.first = host.size();
+        uint64_t unit = 0;
+        for (int idx : g.order) {
+            BatchJob j = b->jobs[idx].job;
+            if (g.interp >= 0) {
+                const double rounds = std::round((double)t.unit_kops * 1000.0 / b->jobs[idx].block_ops);
+                j.unit_blocks = 4u * (uint32_t)std::min<double>(std::max(rounds, 1.0), (double)kMaxRounds);
+            } else {
+                j.unit_blocks = kLightUnitBlocks;
+            }
+            j.unit_begin = (uint32_t)unit;
+            unit += (j.nblocks + j.unit_blocks - 1) / j.unit_blocks;
+            if (unit >= (1ull << 31)) return fail(QTX_ERANGE, "batch has more than 2^31 work units");
+            host.push_back(j);
+        }
+        g.nunits = (uint32_t)unit;
+        if (g.interp >= 0) {
+            const long long waves_needed = ((long long)g.nunits + 3) / 4;  // four waves per workgroup
+            g.grid = (unsigned)std::max(1LL, std::min<long long>((long long)b->ctx->cu_count * t.workgroups_per_cu,
+                                                                 waves_needed));
+        } else {
+            g.grid = g.nunits;
+        }
+    }
+    if (b->done) QTX_HIP(hipEventSynchronize(b->done));
+    QTX_HIP(hipMemcpy(b->d_jobs, host.data(), host.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+    drop_graph(b);
+
+    QTX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(b->fork, b->cap);
+    for (size_t i = 0; i < b->groups.size() && e == hipSuccess; i++) {
+        const auto &g = b->groups[i];
+        hipStream_t s = i == 0 ? b->cap : b->side[i - 1];
+        if (i > 0) e = hipStreamWaitEvent(s, b->fork, 0);
+        if (e != hipSuccess) break;
+        const BatchJob *dj = b->d_jobs + g.first;
+        if (g.interp >= 0)
+            e = launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, b->d_queues + 2 * i, g.grid,
+                                   b->ctx->d_tables, s);
+        else
+            e = launch_light_batch(dj, (int)g.order.size(), g.nunits, s);
+        if (e == hipSuccess && i > 0) e = hipEventRecord(b->joins[i - 1], s);
+    }
+    for (size_t i = 1; i < b->groups.size() && e == hipSuccess; i++) e = hipStreamWaitEvent(b->cap, b->joins[i - 1], 0);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->cap, &graph);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(e, "batch capture");
+    }
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    b->graph = graph;
+    QTX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return QTX_OK;
+}
+
+qtx_status check_tuning(const qtx_batch_tuning *t) {
+    if (!t) return fail(QTX_EINVAL, "null tuning");
+    if (t->workgroups_per_cu < 1 || t->workgroups_per_cu > kMaxColourWgPerCu)
+        return fail(QTX_EINVAL, "workgroups_per_cu must be in 1..%d", kMaxColourWgPerCu);
+    if (t->unit_kops < 16 || t->unit_kops > 16384) return fail(QTX_EINVAL, "unit_kops must be in 16..16384");
+    return QTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+qtx_status qtx_batch_destroy(qtx_batch *b) {
+    if (!b) return QTX_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->done) (void)hipEventSynchronize(b->done);
+    drop_graph(b);
+    for (hipEvent_t ev : b->joins) (void)hipEventDestroy(ev);
+    for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_queues) (void)hipFree(b->d_queues);
+    delete b;
+    return QTX_OK;
+}
+
+qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx_batch **out) {
+    if (!out) return fail(QTX_EINVAL, "null output pointer");
+    *out = nullptr;
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    if (!jobs || njobs == 0) return fail(QTX_EINVAL, "empty batch");
+    if (njobs > (1u << 24)) return fail(QTX_ERANGE, "batch of %zu jobs exceeds 2^24", njobs);
+    auto *b = new qtx_batch;
+    b->ctx = ctx;
+    b->jobs.resize(njobs);
+    for (size_t i = 0; i < njobs; i++) {
+        if ((st = prepare(jobs[i], i, b->jobs[i])) != QTX_OK) {
+            delete b;
+            return st;
+        }
+    }
+    // groups: colour jobs by interpolator, then light jobs; each in descending total cost
+    std::vector<int> idx(njobs);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto total = [&](int i) { return b->jobs[i].block_ops * b->jobs[i].job.nblocks; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return total(x) > total(y); });
+    for (int grp = 0; grp <= 4; grp++) {
+        qtx_batch::Group g;
+        g.interp = grp < 4 ? grp : -1;
+        for (int i : idx)
+            if (b->jobs[i].group == grp) g.order.push_back(i);
+        if (!g.order.empty()) b->groups.push_back(std::move(g));
+    }
+    hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    for (size_t i = 1; i < b->groups.size() && e == hipSuccess; i++) {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) b->side.push_back(s);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(ev);
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, njobs * sizeof(BatchJob));
+    if (e == hipSuccess) e = hipMalloc(&b->d_queues, 2 * b->groups.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b->d_queues, 0, 2 * b->groups.size() * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        qtx_batch_destroy(b);
+        return hip_fail(e, "batch set-up");
+    }
+    if ((st = rebuild(b)) != QTX_OK) {
+        qtx_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return QTX_OK;
+}
+
+qtx_status qtx_batch_launch(qtx_batch *b, void *stream) {
+    if (!b) return fail(QTX_EINVAL, "null batch");
+    QTX_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    // the work queues belong to the batch: runs of one batch never overlap
+    QTX_HIP(hipStreamWaitEvent(s, b->done, 0));
+    QTX_HIP(hipGraphLaunch(b->exec, s));
+    QTX_HIP(hipEventRecord(b->done, s));
+    return QTX_OK;
+}
+
+qtx_status qtx_batch_set_tuning(qtx_batch *b, const qtx_batch_tuning *t) {
+    if (!b) return fail(QTX_EINVAL, "null batch");
+    qtx_status st = check_tuning(t);
+    if (st != QTX_OK) return st;
+    QTX_HIP(hipSetDevice(b->ctx->device));
+    b->tuning = *t;
+    return rebuild(b);
+}
+
+qtx_status qtx_batch_get_tuning(const qtx_batch *b, qtx_batch_tuning *t) {
+    if (!b || !t) return fail(QTX_EINVAL, "null argument");
+    *t = b->tuning;
+    return QTX_OK;
+}
+
+qtx_status qtx_batch_autotune(qtx_batch *b, void *stream, int32_t repeats, qtx_batch_tuning *best, float *best_ms) {
+    if (!b) return fail(QTX_EINVAL, "null batch");
+    if (repeats < 1) return fail(QTX_EINVAL, "repeats must be >= 1");
+    QTX_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    static const int wgs[] = {2, 3, 4};
+    static const uint32_t kops[] = {64, 256, 1024, 4096};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    QTX_HIP(hipEventCreate(&t0));
+    hipError_t e = hipEventCreate(&t1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    qtx_batch_tuning chosen = b->tuning;
+    float chosen_ms = INFINITY;
+    qtx_status st = QTX_OK;
+    for (int w : wgs) {
+        for (uint32_t k : kops) {
+            b->tuning = qtx_batch_tuning{w, k};
+            if ((st = rebuild(b)) != QTX_OK) break;
+            if ((st = qtx_batch_launch(b, stream)) != QTX_OK) break;  // warm-up
+            e = hipEventRecord(t0, s);
+            for (int r = 0; r < repeats && st == QTX_OK && e == hipSuccess; r++) st = qtx_batch_launch(b, stream);
+            if (st != QTX_OK) break;
+            if (e == hipSuccess) e = hipEventRecord(t1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "autotune timing");
+                break;
+            }
+            ms /= (float)repeats;
+            if (ms < chosen_ms) {
+                chosen_ms = ms;
+                chosen = b->tuning;
+            }
+        }
+        if (st != QTX_OK) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (st != QTX_OK) return st;
+    b->tuning = chosen;
+    if ((st = rebuild(b)) != QTX_OK) return st;
+    if (best) *best = chosen;
+    if (best_ms) *best_ms = chosen_ms;
+    return QTX_OK;
+}
+
+qtx_status qtx_batch_run(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, void *stream) {
+    qtx_batch *b = nullptr;
+    qtx_status st = qtx_batch_create(ctx, jobs, njobs, &b);
+    if (st != QTX_OK) return st;
+    st = qtx_batch_launch(b, stream);
+    if (st == QTX_OK) {
+        const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+    }
+    qtx_batch_destroy(b);
+    return st;
+}
+
+qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *dst_bytes, size_t njobs) {
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    if (!jobs || !dst_bytes || njobs == 0) return fail(QTX_EINVAL, "empty batch");
+    // sizes first (validates formats and dimensions)
+    std::vector<Prepared> prep(njobs);
+    std::vector<size_t> src_off(njobs), dst_off(njobs);
+    size_t total = 0;
+    auto align = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    for (size_t i = 0; i < njobs; i++) {
+        if ((st = prepare(jobs[i], i, prep[i])) != QTX_OK) return st;
+        if (dst_bytes[i] < prep[i].dst_bytes)
+            return fail(QTX_EINVAL, "job %zu: output buffer holds %zu bytes, %zu needed", i, dst_bytes[i],
+                        prep[i].dst_bytes);
+        src_off[i] = total;
+        total += align(prep[i].src_bytes);
+    }
+    for (size_t i = 0; i < njobs; i++) {
+        dst_off[i] = total;
+        total += align(prep[i].dst_bytes);
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
+    std::vector<qtx_job> dev(jobs, jobs + njobs);
+    for (size_t i = 0; i < njobs; i++) {
+        QTX_HIP(hipMemcpyAsync(ctx->d_in + src_off[i], jobs[i].src, prep[i].src_bytes, hipMemcpyHostToDevice,
+                               ctx->stream));
+        dev[i].src = ctx->d_in + src_off[i];
+        dev[i].dst = ctx->d_in + dst_off[i];
+    }
+    qtx_batch *b = nullptr;
+    if ((st = qtx_batch_create(ctx, dev.data(), njobs, &b)) != QTX_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st;
+    }
+    st = qtx_batch_launch(b, ctx->stream);
+    for (size_t i = 0; i < njobs && st == QTX_OK; i++) {
+        const hipError_t e = hipMemcpyAsync(jobs[i].dst, ctx->d_in + dst_off[i], prep[i].dst_bytes,
+                                            hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpyAsync");
+    }
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+    qtx_batch_destroy(b);
+    return st;
+}
+
+}  // extern "C"

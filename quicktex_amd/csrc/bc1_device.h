@@ -191,7 +191,10 @@ template <int INTERP> __device__ __forceinline__ void palette_pair(uint32_t ep, 
 
 // ---------------------------------------------------------------------------
 // Per-workgroup LDS: tables copied from DeviceTables once, then the texels of
-// the workgroup's 16 blocks (16 words per group) and the launch's settings.
+// the workgroup's 16 blocks (16 words per group) and one settings slot per wave
+// (a wave's four blocks always share their settings; waves of one workgroup may
+// work on differently configured textures in a batch).
+constexpr int kWaves = 4;  // waves per 256-lane workgroup
 struct Lds {
     float4 f4[kOrders4];  // LS factors per ordering
     float4 f3[kOrders3];
@@ -200,7 +203,7 @@ struct Lds {
     uint32_t lv3[kOrders3];
     float mid5[32], mid6[64];
     uint32_t match[2][256];  // this launch's interpolator, 5- and 6-bit
-    BC1Settings s;
+    BC1Settings s[kWaves];
 };
 extern __shared__ __align__(16) uint8_t qtx_lds_raw[];
 __device__ __forceinline__ Lds &L() { return *reinterpret_cast<Lds *>(qtx_lds_raw); }
@@ -208,7 +211,8 @@ __device__ __forceinline__ unsigned gbase() { return threadIdx.x & ~(unsigned)(G
 __device__ __forceinline__ uint32_t tex(int i) { return L().tex[gbase() + (unsigned)i]; }
 __device__ __forceinline__ uint4 tex4(int q) { return reinterpret_cast<const uint4 *>(&L().tex[gbase()])[q]; }
 __device__ __forceinline__ uint32_t pick(const uint4 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
-__device__ __forceinline__ const BC1Settings &S() { return L().s; }
+__device__ __forceinline__ int wave_slot() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+__device__ __forceinline__ const BC1Settings &S() { return L().s[wave_slot()]; }
 
 struct Ctx {
     const DeviceTables *T;
@@ -878,7 +882,8 @@ template <int INTERP> __device__ __forceinline__ uint2 encode_bc1(const Ctx &cx)
     return write_block(result);
 }
 
-// Copy the launch's tables and settings into LDS (every thread of the workgroup).
+// Copy the launch's tables into LDS and s into every wave's settings slot (every
+// thread of the workgroup takes part).
 template <int INTERP> __device__ __forceinline__ void load_lds_tables(const DeviceTables *T, const BC1Settings &s) {
     Lds &l = L();
     for (int i = (int)threadIdx.x; i < kOrders4; i += (int)blockDim.x) {
@@ -892,7 +897,7 @@ template <int INTERP> __device__ __forceinline__ void load_lds_tables(const Devi
     for (int i = (int)threadIdx.x; i < 512; i += (int)blockDim.x) l.match[i >> 8][i & 255] = T->match[INTERP][i >> 8][i & 255];
     if (threadIdx.x < 32) l.mid5[threadIdx.x] = T->mid5[threadIdx.x];
     if (threadIdx.x < 64) l.mid6[threadIdx.x] = T->mid6[threadIdx.x];
-    if (threadIdx.x == 0) l.s = s;
+    if (threadIdx.x < kWaves) l.s[threadIdx.x] = s;
 }
 
 }  // namespace dev

@@ -10,22 +10,16 @@
 
 #include "kernels.h"
 #include "qtx_internal.h"
+#include "runtime.h"
 
-struct qtx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    qtx::DeviceTables *d_tables = nullptr;
-    // staging for the host entry points
-    uint8_t *d_in = nullptr;
-    size_t in_cap = 0;
-    uint8_t *d_out = nullptr;
-    size_t out_cap = 0;
-    std::mutex mu;
-};
+namespace qtx {
+namespace rt {
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+const char *last_error() { return g_err.c_str(); }
 
 qtx_status fail(qtx_status st, const char *fmt, ...) {
     char buf[512];
@@ -41,12 +35,6 @@ qtx_status hip_fail(hipError_t e, const char *what) {
     return fail(QTX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define QTX_HIP(call)                                         \
-    do {                                                      \
-        hipError_t e_ = (call);                               \
-        if (e_ != hipSuccess) return hip_fail(e_, #call);     \
-    } while (0)
-
 qtx_status check_dims(int32_t w, int32_t h) {
     // RawTexture/BlockTexture constructors (quicktex/Texture.h:56-57)
     if (w <= 0) return fail(QTX_EINVAL, "Texture width must be greater than 0");
@@ -61,13 +49,15 @@ size_t block_bytes(int32_t w, int32_t h, int bytes_per_block) {
     return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * (size_t)bytes_per_block;
 }
 
+size_t raw_bytes(int32_t w, int32_t h) { return (size_t)w * (size_t)h * 4; }
+
 qtx_status check_ctx(qtx_ctx *ctx) {
     if (!ctx) return fail(QTX_EINVAL, "null context");
     QTX_HIP(hipSetDevice(ctx->device));
     return QTX_OK;
 }
 
-qtx_status to_settings(const qtx_bc1_params *p, qtx::BC1Settings &s, int &interp) {
+qtx_status to_settings(const qtx_bc1_params *p, BC1Settings &s, int &interp) {
     if (!p) return fail(QTX_EINVAL, "null params");
     qtx_bc1_params c = *p;
     qtx_status st = qtx_bc1_params_clamp(&c);
@@ -101,11 +91,14 @@ qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need) {
     return QTX_OK;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace qtx
+
+using namespace qtx::rt;
 
 extern "C" {
 
-const char *qtx_last_error(void) { return g_err.c_str(); }
+const char *qtx_last_error(void) { return last_error(); }
 
 qtx_status qtx_bc1_params_clamp(qtx_bc1_params *p) {
     if (!p) return fail(QTX_EINVAL, "null params");
@@ -182,6 +175,7 @@ qtx_status qtx_ctx_create(int32_t device, qtx_ctx **out) {
     auto *ctx = new qtx_ctx;
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_tables, sizeof(qtx::DeviceTables));
     if (e == hipSuccess)
         e = hipMemcpy(ctx->d_tables, &qtx::host_tables(), sizeof(qtx::DeviceTables), hipMemcpyHostToDevice);
@@ -327,8 +321,6 @@ qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *
     QTX_HIP(hipStreamSynchronize(ctx->stream));
     return QTX_OK;
 }
-
-size_t raw_bytes(int32_t w, int32_t h) { return (size_t)w * (size_t)h * 4; }
 
 }  // namespace
 

@@ -14,4 +14,14 @@ hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
 hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint8_t *d_rgba, int interp, int p0, int p1,
                          hipStream_t stream);
 
+// Batch kernels (jobs sorted by descending cost, unit_begin ascending).
+// Colour (BC1/BC3 encode) jobs of one interpolator: persistent grid taking work
+// units from d_queue (two words, zero before the first launch; the kernel leaves
+// them zero).  Light jobs (BC4/BC5 encode, all decoders): 256 blocks per unit,
+// one workgroup per unit.
+constexpr uint32_t kLightUnitBlocks = 256;
+hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, uint32_t *d_queue,
+                              unsigned grid, const DeviceTables *d_tables, hipStream_t stream);
+hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, hipStream_t stream);
+
 }  // namespace qtx

@@ -70,6 +70,40 @@ __device__ __forceinline__ uint32_t bc4_value(uint32_t a0, uint32_t a1, uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// One BC1 block (the colour half of BC3 as well, with its alpha block when bc3)
+// per group of 16 lanes: gather the texels into the group's LDS words, encode,
+// store.  Every lane of the group calls it with the same gid.
+template <int INTERP>
+__device__ __forceinline__ void encode_color_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                   bool bc3) {
+    const int bx = gid % bw, by = gid / bw;
+    const uint32_t px = load_texel(rgba, w, h, bx, by, glane());
+    L().tex[threadIdx.x] = px;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t rgb0 = px & 0x00FFFFFFu;
+    const int pp = (int)gsum(dot4(rgb0, rgb0));
+    const uint32_t ppa = gsum(dot4(px, px));
+    const Ctx cx{T, px, pp, ppa};
+    const uint2 colour = encode_bc1<INTERP>(cx);
+    uint2 alpha = make_uint2(0, 0);
+    if (bc3) {
+        uint32_t a[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) a[i] = tex(i) >> 24;
+        alpha = bc4_encode_values(a);
+    }
+    if (glane() == 0) {
+        if (bc3) reinterpret_cast<uint4 *>(out)[gid] = make_uint4(alpha.x, alpha.y, colour.x, colour.y);
+        else reinterpret_cast<uint2 *>(out)[gid] = colour;
+    }
+    // the group's next block overwrites these LDS words: keep the reads above them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int INTERP, bool BC3>
 __global__ __launch_bounds__(256, 4) void k_bc1_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                        uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
@@ -83,40 +117,14 @@ __global__ __launch_bounds__(256, 4) void k_bc1_encode(const uint8_t *__restrict
     for (int chunk = (int)blockIdx.x; chunk < nchunks; chunk += (int)gridDim.x) {
         const int gid = chunk * 16 + group;
         if (gid >= nblocks) break;
-        const int bx = gid % bw, by = gid / bw;
-        const uint32_t px = load_texel(rgba, w, h, bx, by, glane());
-        L().tex[threadIdx.x] = px;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t rgb0 = px & 0x00FFFFFFu;
-        const int pp = (int)gsum(dot4(rgb0, rgb0));
-        const uint32_t ppa = gsum(dot4(px, px));
-        const Ctx cx{T, px, pp, ppa};
-        const uint2 colour = encode_bc1<INTERP>(cx);
-        uint2 alpha = make_uint2(0, 0);
-        if (BC3) {
-            uint32_t a[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) a[i] = tex(i) >> 24;
-            alpha = bc4_encode_values(a);
-        }
-        if (glane() == 0) {
-            if (BC3) reinterpret_cast<uint4 *>(out)[gid] = make_uint4(alpha.x, alpha.y, colour.x, colour.y);
-            else reinterpret_cast<uint2 *>(out)[gid] = colour;
-        }
-        // the next chunk's texels overwrite this group's LDS words: keep the reads above them
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        encode_color_block<INTERP>(rgba, w, h, bw, gid, out, T, BC3);
     }
 }
 
-// BC4 (one channel) or BC5 (two channels): one lane per block.
+// BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
 template <bool BC5>
-__global__ __launch_bounds__(256) void k_bc4_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
-                                                    int ch0, int ch1, uint8_t *__restrict__ out) {
-    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (gid >= nblocks) return;
+__device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid, int ch0,
+                                                  int ch1, uint8_t *__restrict__ out) {
     const int bx = gid % bw, by = gid / bw;
     uint32_t px[16];
     if ((w & 3) == 0 && bx * 4 + 3 < w && by * 4 + 3 < h) {
@@ -143,6 +151,13 @@ __global__ __launch_bounds__(256) void k_bc4_encode(const uint8_t *__restrict__ 
     }
 }
 
+template <bool BC5>
+__global__ __launch_bounds__(256) void k_bc4_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
+                                                    int ch0, int ch1, uint8_t *__restrict__ out) {
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gid < nblocks) encode_bc45_block<BC5>(rgba, w, h, bw, gid, ch0, ch1, out);
+}
+
 // ---------------------------------------------------------------------------
 // Decoders: one lane per block.
 enum DecKind { DEC_BC1 = 1, DEC_BC3 = 3, DEC_BC4 = 4, DEC_BC5 = 5 };
@@ -159,11 +174,10 @@ __device__ __forceinline__ bool axis_winner(int b, int t, int n, int nb) {
     return b == (X >> 2) && t == (X & 3);
 }
 
+// Decode of block gid of a bw x bh block texture into the w x h RGBA8 texture.
 template <int INTERP, int KIND>
-__global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, int w, int h, int bw, int bh,
-                                                int p0, int p1, uint8_t *__restrict__ rgba) {
-    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (gid >= bw * bh) return;
+__device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks, int w, int h, int bw, int bh, int p0,
+                                             int p1, uint8_t *__restrict__ rgba, int gid) {
     const int bx = gid % bw, by = gid / bw;
     uint32_t px[16];
 #pragma unroll
@@ -256,6 +270,131 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ bloc
     }
 }
 
+template <int INTERP, int KIND>
+__global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, int w, int h, int bw, int bh,
+                                                int p0, int p1, uint8_t *__restrict__ rgba) {
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gid < bw * bh) decode_block<INTERP, KIND>(blocks, w, h, bw, bh, p0, p1, rgba, gid);
+}
+
+// ---------------------------------------------------------------------------
+// Batch kernels.  A batch is a list of jobs (textures); each kernel of a batch
+// owns a job array sorted by descending cost and a queue of work units numbered
+// job by job.
+
+// Job of unit u: the last job whose unit_begin <= u, searched upward from a
+// job at or before it (units are taken in increasing order, so a wave's next
+// unit is in its current job or a later one).
+__device__ __forceinline__ int find_job(const BatchJob *__restrict__ jobs, int from, int njobs, uint32_t u) {
+    if (from + 1 >= njobs || jobs[from + 1].unit_begin > u) return from;
+    int lo = from + 1, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].unit_begin <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Next unit of the launch's queue for the calling wave (one atomic per wave).
+__device__ __forceinline__ uint32_t take_unit(uint32_t *queue) {
+    uint32_t u = 0;
+    if ((threadIdx.x & 63) == 0) u = atomicAdd(&queue[0], 1u);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)u);
+}
+
+// queue[0] hands out units and queue[1] counts the waves that found it empty;
+// the last of those waves returns both to zero, so the next launch of the same
+// kernel on the same queue (stream-ordered after this one) starts from zero with
+// no fill or memset in between.  Every wave calls this exactly once.
+__device__ __forceinline__ void retire(uint32_t *queue, uint32_t waves) {
+    if ((threadIdx.x & 63) == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == waves - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// Persistent BC1/BC3 encoder over the jobs of one interpolator.  Each wave takes
+// a unit (unit_blocks consecutive blocks of one job), loads the job's settings
+// into its LDS slot when the job changes, and encodes the unit four blocks at a
+// time (one per group).
+template <int INTERP>
+__global__ __launch_bounds__(256, 4) void k_color_batch(const BatchJob *__restrict__ jobs, int njobs, uint32_t nunits,
+                                                        uint32_t *__restrict__ queue,
+                                                        const DeviceTables *__restrict__ T) {
+    load_lds_tables<INTERP>(T, jobs[0].s);
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63);
+    const int slot = wave_slot();
+    const uint32_t sub = (threadIdx.x & 63) >> 4;  // group within the wave
+    int cur = 0;
+    for (;;) {
+        const uint32_t u = take_unit(queue);
+        if (u >= nunits) break;
+        const int j = find_job(jobs, cur, njobs, u);
+        if (j != cur) {
+            // all reads of the slot by this wave were issued before this point, and
+            // one wave's LDS operations complete in order
+            if (lane < (int)(sizeof(BC1Settings) / 4))
+                reinterpret_cast<uint32_t *>(&L().s[slot])[lane] = reinterpret_cast<const uint32_t *>(&jobs[j].s)[lane];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            cur = j;
+        }
+        const BatchJob &J = jobs[j];
+        const uint32_t b0 = (u - J.unit_begin) * J.unit_blocks;
+        const uint32_t b1 = min(b0 + J.unit_blocks, J.nblocks);
+        const bool bc3 = J.kind == BJ_ENC_BC3;
+        for (uint32_t b = b0; b < b1; b += 4) {
+            const uint32_t gid = b + sub;
+            if (gid < b1) encode_color_block<INTERP>(J.src, J.w, J.h, J.bw, (int)gid, J.dst, T, bc3);
+        }
+    }
+    retire(queue, gridDim.x * kWaves);
+}
+
+// BC4/BC5 encode and every decoder of a batch: one lane per block, one unit of
+// 256 blocks per workgroup.
+template <int KIND>
+__device__ __forceinline__ void decode_interp(const BatchJob &J, int gid) {
+    switch (J.interp) {
+        case QTX_INTERP_IDEAL_ROUND:
+            decode_block<QTX_INTERP_IDEAL_ROUND, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid);
+            break;
+        case QTX_INTERP_NVIDIA:
+            decode_block<QTX_INTERP_NVIDIA, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid);
+            break;
+        case QTX_INTERP_AMD: decode_block<QTX_INTERP_AMD, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid); break;
+        default: decode_block<QTX_INTERP_IDEAL, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid); break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict__ jobs, int njobs) {
+    const uint32_t u = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].unit_begin <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchJob &J = jobs[lo];
+    const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
+    if (gid >= J.nblocks) return;
+    switch (J.kind) {
+        case BJ_ENC_BC4: encode_bc45_block<false>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
+        case BJ_ENC_BC5: encode_bc45_block<true>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
+        case BJ_DEC_BC1: decode_interp<DEC_BC1>(J, (int)gid); break;
+        case BJ_DEC_BC3: decode_interp<DEC_BC3>(J, (int)gid); break;
+        case BJ_DEC_BC4: decode_block<0, DEC_BC4>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
+        case BJ_DEC_BC5: decode_block<0, DEC_BC5>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
+        default: break;
+    }
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -329,6 +468,30 @@ hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint
         default: return hipErrorInvalidValue;
     }
 #undef QTX_DEC
+    return hipGetLastError();
+}
+
+hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, uint32_t *d_queue,
+                              unsigned grid, const DeviceTables *d_tables, hipStream_t stream) {
+    const dim3 block(kThreads);
+#define QTX_CB(I)                                                                                                   \
+    case I:                                                                                                         \
+        hipLaunchKernelGGL((dev::k_color_batch<I>), dim3(grid), block, kBc1Lds, stream, d_jobs, njobs, nunits, d_queue, \
+                           d_tables);                                                                               \
+        break;
+    switch (interp) {
+        QTX_CB(QTX_INTERP_IDEAL)
+        QTX_CB(QTX_INTERP_IDEAL_ROUND)
+        QTX_CB(QTX_INTERP_NVIDIA)
+        QTX_CB(QTX_INTERP_AMD)
+        default: return hipErrorInvalidValue;
+    }
+#undef QTX_CB
+    return hipGetLastError();
+}
+
+hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, hipStream_t stream) {
+    hipLaunchKernelGGL(dev::k_light_batch, dim3(nunits), dim3(kThreads), 0, stream, d_jobs, njobs);
     return hipGetLastError();
 }
 

@@ -48,6 +48,25 @@ struct BC1Settings {
     uint32_t orderings3;
 };
 
+// One texture of a batch as the batch kernels see it (device-resident array,
+// sorted so that the work units of job k are [unit_begin, next job's unit_begin)).
+enum BatchKind : int32_t {
+    BJ_ENC_BC1 = 0x01, BJ_ENC_BC3 = 0x03, BJ_ENC_BC4 = 0x04, BJ_ENC_BC5 = 0x05,
+    BJ_DEC_BC1 = 0x11, BJ_DEC_BC3 = 0x13, BJ_DEC_BC4 = 0x14, BJ_DEC_BC5 = 0x15,
+};
+struct BatchJob {
+    const uint8_t *src;  // RGBA8 texture (encode) or blocks (decode)
+    uint8_t *dst;        // blocks (encode) or RGBA8 texture (decode)
+    int32_t w, h, bw, bh;
+    uint32_t nblocks;      // bw * bh
+    uint32_t unit_begin;   // first work unit of this job within its kernel's queue
+    uint32_t unit_blocks;  // blocks per work unit (a multiple of 4 for colour jobs)
+    int32_t kind;          // BatchKind
+    int32_t interp;        // BC1/BC3 interpolator
+    int32_t p0, p1;        // BC4: channel; BC5: channels; BC1 decode: write_alpha
+    BC1Settings s;         // BC1/BC3 encode
+};
+
 // Built once per process on the host; uploaded per context.
 const DeviceTables &host_tables();
 

@@ -1,0 +1,51 @@
+// runtime.h -- host-side internals shared by the C ABI translation units
+// (capi.cpp, batch.cpp, server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+
+#include "qtx_internal.h"
+
+// A device, its work stream, its copy of the constant tables and the staging
+// buffers of the host-memory entry points.
+struct qtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    qtx::DeviceTables *d_tables = nullptr;
+    int cu_count = 0;
+    // staging for the host entry points
+    uint8_t *d_in = nullptr;
+    size_t in_cap = 0;
+    uint8_t *d_out = nullptr;
+    size_t out_cap = 0;
+    std::mutex mu;
+};
+
+namespace qtx {
+namespace rt {
+
+// Thread-local message for qtx_last_error(); returns st.
+qtx_status fail(qtx_status st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+qtx_status hip_fail(hipError_t e, const char *what);
+const char *last_error();
+
+qtx_status check_dims(int32_t w, int32_t h);
+qtx_status check_ctx(qtx_ctx *ctx);
+// qtx_bc1_params (clamped) -> kernel settings and interpolator
+qtx_status to_settings(const qtx_bc1_params *p, BC1Settings &s, int &interp);
+// grow a device buffer to at least need bytes (contents are not kept)
+qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need);
+size_t block_bytes(int32_t w, int32_t h, int bytes_per_block);
+size_t raw_bytes(int32_t w, int32_t h);
+
+}  // namespace rt
+}  // namespace qtx
+
+#define QTX_HIP(call)                                                   \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return ::qtx::rt::hip_fail(e_, #call);    \
+    } while (0)

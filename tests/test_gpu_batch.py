@@ -1,0 +1,262 @@
+"""GPU tests of the batched form (qtx_batch_* in include/qtx.h).
+
+Every job of a batch must produce the bytes that the plain entry point gives
+for the same texture alone, and those of the CPU checker (the reference's own
+C++ where built, else the oracle): mixed formats, encode and decode, every
+interpolator, colour modes and levels 0-19, ragged sizes down to 1x1, and a
+full mip chain.  The same batch is relaunched under every launch shape and
+after autotuning; each run must give identical bytes.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from quicktex_amd import hip, synth
+from quicktex_amd.capi import (FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5, OP_DECODE, OP_ENCODE, Batch, QtxError, job,
+                               run_host_batch)
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(1, 1), (3, 5), (4, 4), (17, 9), (64, 64), (130, 66), (256, 32), (33, 200), (8, 2), (100, 100), (5, 4),
+         (96, 12)]
+BPB = {FMT_BC1: 8, FMT_BC3: 16, FMT_BC4: 8, FMT_BC5: 16}
+
+
+@pytest.fixture(scope='module')
+def checker():
+    from tests._cpu_libs import oracle, reference
+    return reference() or oracle()
+
+
+@pytest.fixture(scope='module')
+def stream(qtx):
+    qtx.ctx  # the library's context sets the device up first
+    s = hip.Stream()
+    yield s
+    s.destroy()
+
+
+def nblocks(w, h):
+    return ((w + 3) // 4) * ((h + 3) // 4)
+
+
+class Case:
+    """One job: host input, its expected output and device buffers."""
+
+    def __init__(self, i, rng, qtx):
+        w, h = SIZES[i % len(SIZES)]
+        self.w, self.h = w, h
+        self.fmt = (FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5)[i % 4]
+        self.op = OP_DECODE if (i // 4) % 3 == 2 else OP_ENCODE
+        self.interp = int(rng.integers(0, 4))
+        self.level = int(rng.integers(0, 20)) if w * h <= 64 * 64 else int(rng.integers(0, 19))
+        self.mode = 4 if self.fmt == FMT_BC3 else int(rng.choice([4, 3, 0x13]))
+        self.ch0 = int(rng.integers(0, 4))
+        self.ch1 = int(rng.integers(0, 4))
+        self.write_alpha = bool(rng.integers(0, 2))
+        self.params = qtx.params(self.level, self.mode, self.interp)
+        nb = nblocks(w, h)
+        if self.op == OP_ENCODE:
+            img = synth.mixture_rgba(w, h, seed=0x5EED1000 + i, near_black=0.2)
+            img[..., 3] = rng.integers(0, 256, (h, w), dtype=np.uint8) if self.fmt != FMT_BC1 else 255
+            self.src = img
+            self.out_bytes = nb * BPB[self.fmt]
+        else:
+            self.src = rng.integers(0, 256, nb * BPB[self.fmt], dtype=np.uint8)
+            self.out_bytes = w * h * 4
+
+    def cpu(self, checker, oracle):
+        c = oracle if (self.level == 1 and self.interp == 2) else checker
+        if self.op == OP_ENCODE:
+            if self.fmt == FMT_BC1:
+                return c.bc1_encode(self.src, self.params)
+            if self.fmt == FMT_BC3:
+                return c.bc3_encode(self.src, self.params)
+            if self.fmt == FMT_BC4:
+                return c.bc4_encode(self.src, self.ch0)
+            return c.bc5_encode(self.src, self.ch0, self.ch1)
+        if self.fmt == FMT_BC1:
+            return c.bc1_decode(self.src, self.w, self.h, self.interp, self.write_alpha)
+        if self.fmt == FMT_BC3:
+            return c.bc3_decode(self.src, self.w, self.h, self.interp)
+        if self.fmt == FMT_BC4:
+            return c.bc4_decode(self.src, self.w, self.h, self.ch0)
+        return c.bc5_decode(self.src, self.w, self.h, self.ch0, self.ch1)
+
+    def job(self, d_src, d_dst):
+        return job(self.op, self.fmt, self.w, self.h, d_src, d_dst, params=self.params, chan0=self.ch0,
+                   chan1=self.ch1, write_alpha=self.write_alpha, interpolator=self.interp)
+
+    def plain(self, qtx, d_src, d_dst, stream):
+        """The plain (one texture per launch) device entry point."""
+        w, h = self.w, self.h
+        if self.op == OP_ENCODE:
+            if self.fmt == FMT_BC1:
+                qtx.bc1_encode_device(self.params, d_src, w, h, d_dst, stream)
+            elif self.fmt == FMT_BC3:
+                qtx.bc3_encode_device(self.params, d_src, w, h, d_dst, stream)
+            elif self.fmt == FMT_BC4:
+                qtx.bc4_encode_device(self.ch0, d_src, w, h, d_dst, stream)
+            else:
+                qtx.bc5_encode_device(self.ch0, self.ch1, d_src, w, h, d_dst, stream)
+        else:
+            if self.fmt == FMT_BC1:
+                qtx.bc1_decode_device(self.interp, self.write_alpha, d_src, w, h, d_dst, stream)
+            elif self.fmt == FMT_BC3:
+                qtx.bc3_decode_device(self.interp, d_src, w, h, d_dst, stream)
+            elif self.fmt == FMT_BC4:
+                qtx.bc4_decode_device(self.ch0, d_src, w, h, d_dst, stream)
+            else:
+                qtx.bc5_decode_device(self.ch0, self.ch1, d_src, w, h, d_dst, stream)
+
+
+@pytest.fixture(scope='module')
+def mixed(qtx, stream, checker, oracle):
+    rng = np.random.default_rng(0xBA7C4)
+    cases = [Case(i, rng, qtx) for i in range(48)]
+    srcs = [hip.Buffer.from_array(np.ascontiguousarray(c.src).reshape(-1)) for c in cases]
+    want = [np.asarray(c.cpu(checker, oracle)).reshape(-1) for c in cases]
+    return cases, srcs, want
+
+
+def _outs(cases):
+    outs = [hip.Buffer(c.out_bytes) for c in cases]
+    for o in outs:
+        o.zero()
+    return outs
+
+
+def _outs_n(sizes):
+    outs = [hip.Buffer(n) for n in sizes]
+    for o in outs:
+        o.zero()
+    return outs
+
+
+def _check(cases, outs, want, what):
+    for k, (c, o, w) in enumerate(zip(cases, outs, want)):
+        got = o.download()
+        assert got.size == w.size
+        assert (got == w).all(), '%s: job %d (%s fmt %d %dx%d level %d mode %d interp %d) differs' % (
+            what, k, 'enc' if c.op == OP_ENCODE else 'dec', c.fmt, c.w, c.h, c.level, c.mode, c.interp)
+
+
+def test_batch_matches_alone_and_checker(qtx, stream, mixed):
+    cases, srcs, want = mixed
+    alone = _outs(cases)
+    for c, s, o in zip(cases, srcs, alone):
+        c.plain(qtx, s.ptr, o.ptr, stream.handle)
+    stream.synchronize()
+    _check(cases, alone, want, 'plain')
+    outs = _outs(cases)
+    b = Batch(qtx, [c.job(s.ptr, o.ptr) for c, s, o in zip(cases, srcs, outs)])
+    b.launch(stream.handle)
+    stream.synchronize()
+    _check(cases, outs, want, 'batch')
+    b.close()
+
+
+def test_batch_relaunch_under_every_tuning(qtx, stream, mixed):
+    cases, srcs, want = mixed
+    outs = _outs(cases)
+    b = Batch(qtx, [c.job(s.ptr, o.ptr) for c, s, o in zip(cases, srcs, outs)])
+    for wg in (1, 2, 3, 4):
+        for kops in (16, 64, 256, 1024, 16384):
+            b.tuning = (wg, kops)
+            assert b.tuning == (wg, kops)
+            for rep in range(2):  # the work queue must be back at zero for the second run
+                for o in outs:
+                    o.zero()
+                b.launch(stream.handle)
+                stream.synchronize()
+                _check(cases, outs, want, 'tuning (%d, %d) run %d' % (wg, kops, rep))
+    b.close()
+
+
+def test_batch_autotune(qtx, stream, mixed):
+    cases, srcs, want = mixed
+    outs = _outs(cases)
+    b = Batch(qtx, [c.job(s.ptr, o.ptr) for c, s, o in zip(cases, srcs, outs)])
+    (wg, kops), ms = b.autotune(stream.handle, repeats=1)
+    assert 1 <= wg <= 4 and kops in (64, 256, 1024, 4096) and ms > 0
+    assert b.tuning == (wg, kops)
+    for o in outs:
+        o.zero()
+    b.launch(stream.handle)
+    stream.synchronize()
+    _check(cases, outs, want, 'autotuned')
+    b.close()
+
+
+def test_batch_run_host(qtx, mixed):
+    cases, _, want = mixed
+    jobs = []
+    outs = []
+    for c in cases:
+        out = np.zeros(c.out_bytes, np.uint8)
+        outs.append(out)
+        jobs.append((c.job(0, 0), np.ascontiguousarray(c.src).reshape(-1), out))
+    run_host_batch(qtx, jobs)
+    for k, (o, w) in enumerate(zip(outs, want)):
+        assert (o == w).all(), 'job %d' % k
+
+
+def mip_chain(img):
+    """Box-filtered mip chain with floor halving down to 1x1 (image_utils.py:9-36 sizes)."""
+    out = [img]
+    while out[-1].shape[0] > 1 or out[-1].shape[1] > 1:
+        a = out[-1].astype(np.int32)
+        h, w = a.shape[:2]
+        nh, nw = max(h // 2, 1), max(w // 2, 1)
+        ys = np.minimum(np.arange(nh) * 2, h - 1)
+        xs = np.minimum(np.arange(nw) * 2, w - 1)
+        ys1 = np.minimum(ys + 1, h - 1)
+        xs1 = np.minimum(xs + 1, w - 1)
+        m = (a[ys][:, xs] + a[ys][:, xs1] + a[ys1][:, xs] + a[ys1][:, xs1] + 2) >> 2
+        out.append(m.astype(np.uint8))
+    return out
+
+
+def test_batch_mip_chain_bc3(qtx, stream, checker):
+    img = synth.mixture_rgba(300, 202, seed=0x5EED2000)
+    img[..., 3] = synth.smooth_channels(300, 202, seed=0x5EED2001, nchan=1)[..., 0]
+    chain = mip_chain(img)
+    p = qtx.params(10, 4, 0)
+    srcs = [hip.Buffer.from_array(m) for m in chain]
+    outs = _outs_n([nblocks(m.shape[1], m.shape[0]) * 16 for m in chain])
+    jobs = [job(OP_ENCODE, FMT_BC3, m.shape[1], m.shape[0], s.ptr, o.ptr, params=p)
+            for m, s, o in zip(chain, srcs, outs)]
+    b = Batch(qtx, jobs)
+    b.launch(stream.handle)
+    stream.synchronize()
+    for m, o in zip(chain, outs):
+        assert (o.download() == checker.bc3_encode(m, p)).all(), m.shape
+    b.close()
+
+
+def test_batch_errors(qtx, stream):
+    d = hip.Buffer(64 * 64 * 4)
+    p3 = qtx.params(5, 3, 0)
+    good = job(OP_ENCODE, FMT_BC1, 16, 16, d.ptr, d.ptr, params=qtx.params(5))
+    bad = [
+        (job(OP_ENCODE, 2, 16, 16, d.ptr, d.ptr), 'unknown format'),
+        (job(OP_ENCODE, FMT_BC3, 16, 16, d.ptr, d.ptr, params=p3), 'FourColor'),
+        (job(OP_ENCODE, FMT_BC1, 0, 16, d.ptr, d.ptr, params=p3), 'width'),
+        (job(OP_ENCODE, FMT_BC1, 16, 16, 0, d.ptr, params=p3), 'null'),
+        (job(OP_DECODE, FMT_BC4, 16, 16, d.ptr, d.ptr, chan0=4), 'Channel'),
+        (job(OP_DECODE, FMT_BC1, 16, 16, d.ptr, d.ptr, interpolator=7), 'interpolator'),
+        (job(5, FMT_BC1, 16, 16, d.ptr, d.ptr), 'op'),
+    ]
+    for bj, msg in bad:
+        with pytest.raises(QtxError) as e:
+            Batch(qtx, [good, bj])
+        assert e.value.status == 1 and 'job 1' in str(e.value) and msg in str(e.value), str(e.value)
+    with pytest.raises(QtxError):
+        Batch(qtx, [])
+    b = Batch(qtx, [good])
+    for t in ((0, 256), (5, 256), (4, 8), (4, 1 << 20)):
+        with pytest.raises(QtxError):
+            b.tuning = t
+    b.close()
+    assert ctypes.sizeof(type(good)) == 96
