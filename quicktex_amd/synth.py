@@ -19,6 +19,8 @@ Workloads (block mixtures are per 4x4 block):
   replaced by a per-block constant; placed in channels 0..n-1 of RGBA8 with
   the others (0, ..., 255).  This is the 2048x2048 BC4/BC5 workload.
 """
+import os
+
 import numpy as np
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -76,17 +78,32 @@ def _blocks_to_image(blocks, w, h):
     return np.ascontiguousarray(img[:h, :w])
 
 
-def mixture_rgba(w, h, seed=0x5EED0000, texture=0, near_black=0.0):
+_POOL = None
+
+
+def _pool():
+    """Threads for the generators (numpy releases the GIL inside its kernels)."""
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0))))
+    return _POOL
+
+
+def mixture_rgba(w, h, seed=0x5EED0000, texture=0, near_black=0.0, threads=True):
     """(h, w, 4) uint8 RGBA texture of the block mixture."""
     bw, bh = (w + 3) // 4, (h + 3) // 4
-    rows = []
-    step = max(1, 65536 // bw)
-    for y0 in range(0, bh, step):
+    step = max(1, 16384 // bw)
+
+    def rows(y0):
         n = min(step, bh - y0) * bw
         # seeds depend on the absolute block index, so chunking does not change content
         seeds = _block_seeds(seed, texture, y0 * bw + n)[y0 * bw:]
-        rows.append(_mixture_blocks(_Stream(seeds), n, near_black))
-    return _blocks_to_image(np.concatenate(rows), w, h)
+        return _mixture_blocks(_Stream(seeds), n, near_black)
+
+    starts = list(range(0, bh, step))
+    parts = list(_pool().map(rows, starts)) if threads and len(starts) > 1 else [rows(y0) for y0 in starts]
+    return _blocks_to_image(np.concatenate(parts), w, h)
 
 
 def _mixture_blocks(st, nblocks, near_black):
@@ -147,3 +164,105 @@ def smooth_channels(w, h, seed=0x5EED0001, texture=0, nchan=2):
     for c in range(nchan):
         out[:, :, c] = sm[c].astype(np.uint8)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Many textures with mip chains: the batched-encode workload.
+
+def _uniform(seed, texture, k):
+    """Deterministic uniform double in [0, 1) for (seed, texture, k)."""
+    x = _splitmix(np.array([seed], np.uint64))[0] ^ _splitmix(np.array([(texture << 8) + k + 0x77], np.uint64))[0]
+    v = _splitmix(np.array([x], np.uint64))[0]
+    return float(v >> np.uint64(11)) / float(1 << 53)
+
+
+def batch_sizes(n=300, seed=0x5EED0003, lo=16, hi=4096, ragged=0.30):
+    """n (w, h) pairs, each side log-uniform in [lo, hi]; a `ragged` share of the
+    textures has a side that is not a multiple of 4, the rest have both sides
+    rounded down to multiples of 4."""
+    out = []
+    span = np.log2(hi / lo)
+    for t in range(n):
+        w = min(hi, max(lo, int(lo * 2.0 ** (span * _uniform(seed, t, 0)))))
+        h = min(hi, max(lo, int(lo * 2.0 ** (span * _uniform(seed, t, 1)))))
+        if _uniform(seed, t, 2) < ragged:
+            if w % 4 == 0 and h % 4 == 0:
+                d = 1 + int(_uniform(seed, t, 3) * 3)
+                w = w + d if w + d <= hi else w - d
+        else:
+            w, h = w - w % 4, h - h % 4
+        out.append((w, h))
+    return out
+
+
+def alpha_channel(w, h, seed, texture=0, binary=0.15):
+    """(h, w) uint8: uniform noise smoothed by a 3x3 box ((sum + 4) // 9, edge
+    clamp); a `binary` share of the 4x4 blocks instead holds texels of 0 or 255."""
+    words = _block_seeds(seed, texture, w * h)
+    vals = (words & np.uint64(0xFF)).astype(np.int32).reshape(h, w)
+    pad = np.pad(vals, 1, mode='edge')
+    acc = np.zeros_like(vals)
+    for dy in range(3):
+        for dx in range(3):
+            acc += pad[dy:dy + h, dx:dx + w]
+    sm = (acc + 4) // 9
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    bseeds = _block_seeds(seed ^ 0xA1FA, texture, bw * bh)
+    on = ((bseeds % np.uint64(100)) < np.uint64(int(round(binary * 100)))).reshape(bh, bw)
+    on = np.repeat(np.repeat(on, 4, 0), 4, 1)[:h, :w]
+    bit = ((words >> np.uint64(8)) & np.uint64(1)).astype(np.int32).reshape(h, w) * 255
+    return np.where(on, bit, sm).astype(np.uint8)
+
+
+def mip_sizes(w, h):
+    """Mip chain sizes: floor halving down to 1x1 (quicktex/image_utils.py:9-36)."""
+    out = [(w, h)]
+    while out[-1] != (1, 1):
+        pw, ph = out[-1]
+        out.append((max(pw // 2, 1), max(ph // 2, 1)))
+    return out
+
+
+def box_halve(img, nw, nh):
+    """Next mip: 2x2 box (a + b + c + d + 2) >> 2, edge clamp on odd sides."""
+    a = img.astype(np.int32)
+    h, w = a.shape[:2]
+    ys = np.minimum(np.arange(nh) * 2, h - 1)
+    xs = np.minimum(np.arange(nw) * 2, w - 1)
+    ys1, xs1 = np.minimum(ys + 1, h - 1), np.minimum(xs + 1, w - 1)
+    return ((a[ys][:, xs] + a[ys][:, xs1] + a[ys1][:, xs] + a[ys1][:, xs1] + 2) >> 2).astype(np.uint8)
+
+
+def pad4(img):
+    """Pad to multiples of 4 with edge-clamped texels."""
+    h, w = img.shape[:2]
+    ph, pw = (-h) % 4, (-w) % 4
+    if ph == 0 and pw == 0:
+        return img
+    return np.pad(img, ((0, ph), (0, pw), (0, 0)), mode='edge')
+
+
+def batch_textures(n=300, seed=0x5EED0003):
+    """The batched-encode workload: n RGBA8 textures of batch_sizes(), RGB of the
+    block mixture and alpha_channel(), each with its full box-filtered mip chain.
+
+    Returns a list of (texture, level, w, h, rgba) where w x h is the unpadded mip
+    size and rgba the mip padded to multiples of 4 by edge clamp, which is what is
+    encoded."""
+    sizes = batch_sizes(n, seed)
+
+    def one(t):
+        w, h = sizes[t]
+        img = mixture_rgba(w, h, seed=seed, texture=t, threads=False)
+        img[..., 3] = alpha_channel(w, h, seed ^ 0x5A5A, texture=t)
+        out = []
+        for lvl, (mw, mh) in enumerate(mip_sizes(w, h)):
+            if lvl:
+                img = box_halve(img, mw, mh)
+            out.append((t, lvl, mw, mh, pad4(img)))
+        return out
+
+    res = []
+    for chain in _pool().map(one, range(n)):
+        res.extend(chain)
+    return res
