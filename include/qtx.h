@@ -205,6 +205,65 @@ qtx_status qtx_batch_run(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, void *
  * dst_bytes[i] is the capacity of jobs[i].dst. */
 qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *dst_bytes, size_t njobs);
 
+/* ---------------------------------------------------------------------- */
+/* Serving form: a request loop on its own thread.  Requests (encode or
+ * decode, any format, size and settings; host buffers) are admitted in
+ * arrival order as device memory frees up, grouped into batches -- every
+ * request that is waiting and fits joins the next launch -- and run while
+ * earlier batches are still on the device (continuous batching).  Each result
+ * is byte-identical to the plain entry point's for the same request.  This
+ * replaces the reference's one-texture-at-a-time encode()/decode() calls
+ * (quicktex/Encoder.h:36-56, quicktex/Decoder.h:36-54) for callers with
+ * many independent requests. */
+typedef struct qtx_server_config {
+    size_t device_bytes;    /* HBM reserved for request inputs + outputs, the admission
+                               budget (0: 2 GiB)                                         */
+    size_t staging_bytes;   /* pinned host staging for outputs bound for pageable memory
+                               (0: 256 MiB); pinned destinations are written directly   */
+    int32_t slots;          /* batches in flight at once, 1..8 (0: 3)                     */
+    int32_t max_batch_jobs; /* requests per batch, 1..65536 (0: 1024)                     */
+    int32_t max_queue;      /* waiting requests before submit returns EBUSY (0: 65536)    */
+    uint32_t unit_kops;     /* colour work-unit size, as in qtx_batch_tuning; 0: 64        */
+} qtx_server_config;
+
+typedef struct qtx_server_result {
+    int32_t status;     /* qtx_status of the request                                */
+    int32_t batch_jobs; /* requests in the launch that served it                    */
+    double submit_s;    /* server clock (seconds since start): submitted            */
+    double admit_s;     /* device memory reserved                                   */
+    double launch_s;    /* its batch enqueued                                       */
+    double done_s;      /* output in the caller's buffer                            */
+    char message[160];  /* failure description when status != QTX_OK               */
+} qtx_server_result;
+
+typedef struct qtx_server_stats {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t pixels;           /* width * height summed over completed requests   */
+    uint64_t max_waiting;      /* largest number of requests waiting for admission */
+    uint64_t max_batch_jobs;   /* largest batch launched                           */
+    size_t device_bytes_peak;  /* most request memory reserved at once             */
+    double busy_s;             /* time with at least one batch in flight            */
+} qtx_server_stats;
+
+typedef struct qtx_server qtx_server;
+/* Starts the loop on ctx's device (config may be NULL for the defaults). */
+qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_server **out);
+/* Queues one request; job->src/dst are host pointers that must stay valid until
+ * the request completes; dst_bytes is the capacity of job->dst.  Validation
+ * errors are returned here (EINVAL/ERANGE, as the plain entry points); ENOMEM
+ * if the request alone exceeds the device or staging budget; EBUSY if the
+ * queue is full.  Thread-safe. */
+qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes, uint64_t *ticket);
+/* Waits up to timeout_ms (negative: no limit) for the request; QTX_OK and the
+ * result when it is done (the ticket is then released), EBUSY on time-out,
+ * EINVAL for an unknown ticket.  Thread-safe. */
+qtx_status qtx_server_wait(qtx_server *s, uint64_t ticket, int32_t timeout_ms, qtx_server_result *result);
+qtx_status qtx_server_stats_get(qtx_server *s, qtx_server_stats *stats);
+/* Seconds on the server clock now. */
+double qtx_server_now(qtx_server *s);
+/* Stops accepting, finishes every queued request, joins the loop, frees the server. */
+qtx_status qtx_server_stop(qtx_server *s);
+
 #ifdef __cplusplus
 }
 #endif

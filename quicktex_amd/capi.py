@@ -46,6 +46,26 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [('workgroups_per_cu', ctypes.c_int32), ('unit_kops', ctypes.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    """qtx_server_config (include/qtx.h); zero fields take the defaults."""
+    _fields_ = [('device_bytes', ctypes.c_size_t), ('staging_bytes', ctypes.c_size_t), ('slots', ctypes.c_int32),
+                ('max_batch_jobs', ctypes.c_int32), ('max_queue', ctypes.c_int32), ('unit_kops', ctypes.c_uint32)]
+
+
+class ServerResult(ctypes.Structure):
+    """qtx_server_result (include/qtx.h)."""
+    _fields_ = [('status', ctypes.c_int32), ('batch_jobs', ctypes.c_int32), ('submit_s', ctypes.c_double),
+                ('admit_s', ctypes.c_double), ('launch_s', ctypes.c_double), ('done_s', ctypes.c_double),
+                ('message', ctypes.c_char * 160)]
+
+
+class ServerStats(ctypes.Structure):
+    """qtx_server_stats (include/qtx.h)."""
+    _fields_ = [(n, ctypes.c_uint64) for n in ('submitted', 'completed', 'failed', 'batches', 'pixels',
+                                                'max_waiting', 'max_batch_jobs')] + \
+        [('device_bytes_peak', ctypes.c_size_t), ('busy_s', ctypes.c_double)]
+
+
 def job(op, fmt, w, h, src, dst, params=None, chan0=0, chan1=1, write_alpha=False, interpolator=None):
     """A Job; params is a BC1Params for BC1/BC3 encodes, interpolator for BC1/BC3 decodes."""
     j = Job()
@@ -119,6 +139,13 @@ class Qtx:
         L.qtx_batch_destroy.argtypes = [vp]
         L.qtx_batch_run.argtypes = [vp, JP, sz, vp]
         L.qtx_batch_run_host.argtypes = [vp, JP, ctypes.POINTER(sz), sz]
+        L.qtx_server_start.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        L.qtx_server_submit.argtypes = [vp, JP, sz, ctypes.POINTER(ctypes.c_uint64)]
+        L.qtx_server_wait.argtypes = [vp, ctypes.c_uint64, i32, ctypes.POINTER(ServerResult)]
+        L.qtx_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.qtx_server_now.argtypes = [vp]
+        L.qtx_server_now.restype = ctypes.c_double
+        L.qtx_server_stop.argtypes = [vp]
         self._ctx = None
 
     # -- plumbing -------------------------------------------------------------
@@ -267,6 +294,47 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """A qtx_server: submit host-memory requests, wait for their results."""
+
+    def __init__(self, q, device_bytes=0, staging_bytes=0, slots=0, max_batch_jobs=0, max_queue=0, unit_kops=0):
+        self.q = q
+        cfg = ServerConfig(device_bytes, staging_bytes, slots, max_batch_jobs, max_queue, unit_kops)
+        h = ctypes.c_void_p()
+        q.check(q.lib.qtx_server_start(q.ctx, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, j, src, dst):
+        """Queue Job j reading host array src and writing host array dst; returns the ticket."""
+        j.src = src.ctypes.data
+        j.dst = dst.ctypes.data
+        t = ctypes.c_uint64()
+        self.q.check(self.q.lib.qtx_server_submit(self.h, ctypes.byref(j), dst.nbytes, ctypes.byref(t)))
+        self._keep[t.value] = (src, dst)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        r = ServerResult()
+        self.q.check(self.q.lib.qtx_server_wait(self.h, ticket, timeout_ms, ctypes.byref(r)))
+        self._keep.pop(ticket, None)
+        return r
+
+    def stats(self):
+        st = ServerStats()
+        self.q.check(self.q.lib.qtx_server_stats_get(self.h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def now(self):
+        return self.q.lib.qtx_server_now(self.h)
+
+    def stop(self):
+        if self.h is not None:
+            self.q.lib.qtx_server_stop(self.h)
+            self.h = None
+            self._keep.clear()
 
 
 def run_host_batch(q, jobs):

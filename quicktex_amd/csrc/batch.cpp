@@ -16,28 +16,22 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "kernels.h"
 #include "qtx_internal.h"
+#include "batch_internal.h"
 #include "runtime.h"
 
 using namespace qtx;
 using namespace qtx::rt;
 
-namespace {
-
-constexpr int kMaxColourWgPerCu = 4;  // __launch_bounds__(256, 4) of the colour kernel
-constexpr uint32_t kMaxRounds = 1024;  // a colour unit holds at most 4 * kMaxRounds blocks
-
-struct Prepared {
-    BatchJob job;
-    double block_ops;  // estimated ALU ops per block
-    int group;         // interpolator 0..3 for colour jobs, 4 for light jobs
-    size_t src_bytes, dst_bytes;
-};
+namespace qtx {
+namespace bt {
 
 // Estimated ALU ops per BC1 block for the encoder settings: a fit of the
 // oracle's op accounting on the block mixture (levels 0-19): about 2k ops of
@@ -53,13 +47,24 @@ double bc1_block_ops(const BC1Settings &s) {
     return ops;
 }
 
-qtx_status job_fail(size_t i) { return fail(QTX_EINVAL, "job %zu: %s", i, last_error()); }
+// Failure of job i, or of a single request when i is kNoIndex: the message is
+// prefixed with "job <i>: " for batches.
+static qtx_status jfail(size_t i, qtx_status st, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static qtx_status jfail(size_t i, qtx_status st, const char *fmt, ...) {
+    char buf[400];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (i == kNoIndex) return fail(st, "%s", buf);
+    return fail(st, "job %zu: %s", i, buf);
+}
 
 qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
     std::memset(&p, 0, sizeof p);
     qtx_status st = check_dims(j.width, j.height);
-    if (st != QTX_OK) return fail(st, "job %zu: %s", i, last_error());
-    if (!j.src || !j.dst) return fail(QTX_EINVAL, "job %zu: null buffer", i);
+    if (st != QTX_OK) return jfail(i, st, "%s", last_error());
+    if (!j.src || !j.dst) return jfail(i, QTX_EINVAL, "null buffer");
     BatchJob &b = p.job;
     b.src = static_cast<const uint8_t *>(j.src);
     b.dst = static_cast<uint8_t *>(j.dst);
@@ -70,22 +75,22 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
     b.nblocks = (uint32_t)b.bw * (uint32_t)b.bh;
     const int bpb = (j.format == QTX_FMT_BC1 || j.format == QTX_FMT_BC4) ? 8 : 16;
     if (j.format != QTX_FMT_BC1 && j.format != QTX_FMT_BC3 && j.format != QTX_FMT_BC4 && j.format != QTX_FMT_BC5)
-        return fail(QTX_EINVAL, "job %zu: unknown format %d", i, j.format);
+        return jfail(i, QTX_EINVAL, "unknown format %d", j.format);
     if (j.op == QTX_OP_ENCODE) {
         p.src_bytes = raw_bytes(j.width, j.height);
         p.dst_bytes = block_bytes(j.width, j.height, bpb);
         if (j.format == QTX_FMT_BC1 || j.format == QTX_FMT_BC3) {
             int interp = 0;
-            if (to_settings(&j.bc1, b.s, interp) != QTX_OK) return job_fail(i);
+            if (to_settings(&j.bc1, b.s, interp) != QTX_OK) return jfail(i, QTX_EINVAL, "%s", last_error());
             if (j.format == QTX_FMT_BC3 && b.s.color_mode != QTX_COLOR_FOUR)
-                return fail(QTX_EINVAL, "job %zu: BC3 colour blocks are always FourColor", i);
+                return jfail(i, QTX_EINVAL, "BC3 colour blocks are always FourColor");
             b.kind = j.format == QTX_FMT_BC1 ? BJ_ENC_BC1 : BJ_ENC_BC3;
             b.interp = interp;
             p.group = interp;
             p.block_ops = bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
         } else {
             if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
-                return fail(QTX_EINVAL, "job %zu: Channel out of range", i);
+                return jfail(i, QTX_EINVAL, "Channel out of range");
             b.kind = j.format == QTX_FMT_BC4 ? BJ_ENC_BC4 : BJ_ENC_BC5;
             b.p0 = j.chan0;
             b.p1 = j.format == QTX_FMT_BC5 ? j.chan1 : 0;
@@ -101,38 +106,84 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
             case QTX_FMT_BC1:
             case QTX_FMT_BC3:
                 if (j.bc1.interpolator < 0 || j.bc1.interpolator > 3)
-                    return fail(QTX_EINVAL, "job %zu: Invalid interpolator type", i);
+                    return jfail(i, QTX_EINVAL, "Invalid interpolator type");
                 b.kind = j.format == QTX_FMT_BC1 ? BJ_DEC_BC1 : BJ_DEC_BC3;
                 b.interp = j.bc1.interpolator;
                 b.p0 = j.format == QTX_FMT_BC1 ? (j.write_alpha != 0) : 0;
                 break;
             default:
                 if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
-                    return fail(QTX_EINVAL, "job %zu: Channel out of range", i);
+                    return jfail(i, QTX_EINVAL, "Channel out of range");
                 b.kind = j.format == QTX_FMT_BC4 ? BJ_DEC_BC4 : BJ_DEC_BC5;
                 b.p0 = j.chan0;
                 b.p1 = j.format == QTX_FMT_BC5 ? j.chan1 : 0;
                 break;
         }
     } else {
-        return fail(QTX_EINVAL, "job %zu: unknown op %d", i, j.op);
+        return jfail(i, QTX_EINVAL, "unknown op %d", j.op);
     }
     return QTX_OK;
 }
 
-}  // namespace
+// Unit layout of a list of prepared jobs: groups (colour by interpolator, then
+// light), each in descending total cost, unit sizes for the tuning.
+qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, Plan &out) {
+    out.host.clear();
+    out.groups.clear();
+    std::vector<int> idx(jobs.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    auto total = [&](int i) { return jobs[i].block_ops * jobs[i].job.nblocks; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return total(x) > total(y); });
+    for (int grp = 0; grp <= 4; grp++) {
+        Plan::Group g;
+        g.interp = grp < 4 ? grp : -1;
+        g.first = out.host.size();
+        uint64_t unit = 0;
+        for (int i : idx) {
+            if (jobs[i].group != grp) continue;
+            BatchJob j = jobs[i].job;
+            if (g.interp >= 0) {
+                const double rounds = std::round((double)t.unit_kops * 1000.0 / jobs[i].block_ops);
+                j.unit_blocks = 4u * (uint32_t)std::min<double>(std::max(rounds, 1.0), (double)kMaxRounds);
+            } else {
+                j.unit_blocks = kLightUnitBlocks;
+            }
+            j.unit_begin = (uint32_t)unit;
+            unit += (j.nblocks + j.unit_blocks - 1) / j.unit_blocks;
+            if (unit >= (1ull << 31)) return fail(QTX_ERANGE, "batch has more than 2^31 work units");
+            out.host.push_back(j);
+            g.order.push_back(i);
+        }
+        if (g.order.empty()) continue;
+        g.nunits = (uint32_t)unit;
+        if (g.interp >= 0) {
+            const long long waves_needed = ((long long)g.nunits + 3) / 4;  // four waves per workgroup
+            g.grid = (unsigned)std::max(1LL, std::min<long long>((long long)cu_count * t.workgroups_per_cu,
+                                                                 waves_needed));
+        } else {
+            g.grid = g.nunits;
+        }
+        out.groups.push_back(std::move(g));
+    }
+    return QTX_OK;
+}
+
+hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t *d_queue,
+                         const DeviceTables *d_tables, hipStream_t s) {
+    const BatchJob *dj = d_jobs + g.first;
+    if (g.interp >= 0) return launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_queue, g.grid, d_tables, s);
+    return launch_light_batch(dj, (int)g.order.size(), g.nunits, s);
+}
+
+}  // namespace bt
+}  // namespace qtx
+
+using namespace qtx::bt;
 
 struct qtx_batch {
-    struct Group {
-        int interp = -1;           // 0..3 colour, -1 light
-        std::vector<int> order;    // job indices, descending total cost
-        size_t first = 0;          // offset into the device job array
-        uint32_t nunits = 0;
-        unsigned grid = 0;
-    };
     qtx_ctx *ctx = nullptr;
     std::vector<Prepared> jobs;
-    std::vector<Group> groups;  // colour groups (by interpolator), then the light group
+    Plan plan;  // colour groups (by interpolator), then the light group
     BatchJob *d_jobs = nullptr;
     uint32_t *d_queues = nullptr;  // two words per group
     qtx_batch_tuning tuning{kMaxColourWgPerCu, 256};
@@ -155,54 +206,24 @@ void drop_graph(qtx_batch *b) {
 
 // Unit layout for the tuning, upload of the job array, capture of the graph.
 qtx_status rebuild(qtx_batch *b) {
-    const qtx_batch_tuning &t = b->tuning;
-    std::vector<BatchJob> host;
-    host.reserve(b->jobs.size());
-    for (auto &g : b->groups) {
-        g.first = host.size();
-        uint64_t unit = 0;
-        for (int idx : g.order) {
-            BatchJob j = b->jobs[idx].job;
-            if (g.interp >= 0) {
-                const double rounds = std::round((double)t.unit_kops * 1000.0 / b->jobs[idx].block_ops);
-                j.unit_blocks = 4u * (uint32_t)std::min<double>(std::max(rounds, 1.0), (double)kMaxRounds);
-            } else {
-                j.unit_blocks = kLightUnitBlocks;
-            }
-            j.unit_begin = (uint32_t)unit;
-            unit += (j.nblocks + j.unit_blocks - 1) / j.unit_blocks;
-            if (unit >= (1ull << 31)) return fail(QTX_ERANGE, "batch has more than 2^31 work units");
-            host.push_back(j);
-        }
-        g.nunits = (uint32_t)unit;
-        if (g.interp >= 0) {
-            const long long waves_needed = ((long long)g.nunits + 3) / 4;  // four waves per workgroup
-            g.grid = (unsigned)std::max(1LL, std::min<long long>((long long)b->ctx->cu_count * t.workgroups_per_cu,
-                                                                 waves_needed));
-        } else {
-            g.grid = g.nunits;
-        }
-    }
+    qtx_status st = plan(b->jobs, b->tuning, b->ctx->cu_count, b->plan);
+    if (st != QTX_OK) return st;
+    const std::vector<BatchJob> &host = b->plan.host;
     if (b->done) QTX_HIP(hipEventSynchronize(b->done));
     QTX_HIP(hipMemcpy(b->d_jobs, host.data(), host.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
     drop_graph(b);
 
     QTX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipEventRecord(b->fork, b->cap);
-    for (size_t i = 0; i < b->groups.size() && e == hipSuccess; i++) {
-        const auto &g = b->groups[i];
+    const auto &groups = b->plan.groups;
+    for (size_t i = 0; i < groups.size() && e == hipSuccess; i++) {
         hipStream_t s = i == 0 ? b->cap : b->side[i - 1];
         if (i > 0) e = hipStreamWaitEvent(s, b->fork, 0);
         if (e != hipSuccess) break;
-        const BatchJob *dj = b->d_jobs + g.first;
-        if (g.interp >= 0)
-            e = launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, b->d_queues + 2 * i, g.grid,
-                                   b->ctx->d_tables, s);
-        else
-            e = launch_light_batch(dj, (int)g.order.size(), g.nunits, s);
+        e = enqueue_group(groups[i], b->d_jobs, b->d_queues + 2 * i, b->ctx->d_tables, s);
         if (e == hipSuccess && i > 0) e = hipEventRecord(b->joins[i - 1], s);
     }
-    for (size_t i = 1; i < b->groups.size() && e == hipSuccess; i++) e = hipStreamWaitEvent(b->cap, b->joins[i - 1], 0);
+    for (size_t i = 1; i < groups.size() && e == hipSuccess; i++) e = hipStreamWaitEvent(b->cap, b->joins[i - 1], 0);
     hipGraph_t graph = nullptr;
     const hipError_t e2 = hipStreamEndCapture(b->cap, &graph);
     if (e != hipSuccess) {
@@ -259,22 +280,15 @@ qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx
             return st;
         }
     }
-    // groups: colour jobs by interpolator, then light jobs; each in descending total cost
-    std::vector<int> idx(njobs);
-    std::iota(idx.begin(), idx.end(), 0);
-    auto total = [&](int i) { return b->jobs[i].block_ops * b->jobs[i].job.nblocks; };
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return total(x) > total(y); });
-    for (int grp = 0; grp <= 4; grp++) {
-        qtx_batch::Group g;
-        g.interp = grp < 4 ? grp : -1;
-        for (int i : idx)
-            if (b->jobs[i].group == grp) g.order.push_back(i);
-        if (!g.order.empty()) b->groups.push_back(std::move(g));
+    if ((st = plan(b->jobs, b->tuning, ctx->cu_count, b->plan)) != QTX_OK) {
+        delete b;
+        return st;
     }
+    const size_t ngroups = b->plan.groups.size();
     hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
-    for (size_t i = 1; i < b->groups.size() && e == hipSuccess; i++) {
+    for (size_t i = 1; i < ngroups && e == hipSuccess; i++) {
         hipStream_t s = nullptr;
         hipEvent_t ev = nullptr;
         e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -283,8 +297,8 @@ qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx
         if (e == hipSuccess) b->joins.push_back(ev);
     }
     if (e == hipSuccess) e = hipMalloc(&b->d_jobs, njobs * sizeof(BatchJob));
-    if (e == hipSuccess) e = hipMalloc(&b->d_queues, 2 * b->groups.size() * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(b->d_queues, 0, 2 * b->groups.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_queues, 2 * ngroups * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b->d_queues, 0, 2 * ngroups * sizeof(uint32_t));
     if (e != hipSuccess) {
         qtx_batch_destroy(b);
         return hip_fail(e, "batch set-up");

@@ -1,0 +1,51 @@
+// batch_internal.h -- job validation and work-unit planning shared by the
+// batched form (batch.cpp) and the serving loop (server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "qtx_internal.h"
+
+namespace qtx {
+namespace bt {
+
+constexpr int kMaxColourWgPerCu = 4;   // __launch_bounds__(256, 4) of the colour kernel
+constexpr uint32_t kMaxRounds = 1024;  // a colour unit holds at most 4 * kMaxRounds blocks
+
+// A validated job: its kernel descriptor, estimated cost and byte sizes.
+struct Prepared {
+    BatchJob job;
+    double block_ops;  // estimated ALU ops per block
+    int group;         // interpolator 0..3 for colour jobs, 4 for light jobs
+    size_t src_bytes, dst_bytes;
+};
+
+// Kernel-ready layout: descriptors in group order, one queue per group.
+struct Plan {
+    struct Group {
+        int interp = -1;         // 0..3 colour, -1 light
+        std::vector<int> order;  // indices into the prepared list, descending total cost
+        size_t first = 0;        // offset of the group's descriptors in host[]
+        uint32_t nunits = 0;
+        unsigned grid = 0;
+    };
+    std::vector<BatchJob> host;
+    std::vector<Group> groups;  // colour groups by interpolator, then the light group
+};
+
+// Estimated ALU ops per BC1 block for the encoder settings.
+double bc1_block_ops(const BC1Settings &s);
+constexpr size_t kNoIndex = ~(size_t)0;
+// Validates job i (messages start "job <i>: "; none for i == kNoIndex) and fills p.
+qtx_status prepare(const qtx_job &j, size_t i, Prepared &p);
+qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, Plan &out);
+// Launches one group's kernel; d_jobs holds plan.host, d_queue the group's two
+// queue words (zero before the first launch; the kernels leave them zero).
+hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t *d_queue,
+                         const DeviceTables *d_tables, hipStream_t s);
+
+}  // namespace bt
+}  // namespace qtx

@@ -1,0 +1,413 @@
+// server.cpp -- the serving form of the codec (include/qtx.h, qtx_server_*).
+//
+// One loop thread per server.  Submitted requests wait in arrival order; when a
+// batch slot is free the loop admits from the front of the queue every request
+// whose input + output fit in the device arena (and, for a destination in
+// pageable memory, in the pinned staging area), stopping at the first that does
+// not fit so that no request is overtaken forever.  The admitted requests form
+// one batch, enqueued on the slot's stream:
+//     H2D of the inputs -> job table -> the batch kernels -> D2H of the outputs
+// and the loop goes straight back to admitting: up to `slots` batches are on
+// the device at once, so one batch's copies overlap another's kernels and
+// requests that arrive while a batch runs join the next one.  A slot whose
+// completion event has fired returns its memory and completes its requests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_internal.h"
+#include "kernels.h"
+#include "qtx_internal.h"
+#include "runtime.h"
+#include "server_core.h"
+
+using namespace qtx;
+using namespace qtx::rt;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr int kQueueWords = 2 * 5;  // two words per kernel group, at most five groups
+
+size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+bool is_pinned(const void *p) {
+    hipPointerAttribute_t a;
+    std::memset(&a, 0, sizeof a);
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory reports an error; clear it
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+struct Req {
+    uint64_t ticket = 0;
+    qtx_job job{};  // host pointers
+    bt::Prepared prep{};
+    bool dst_pinned = false;
+    size_t dev_off = 0, dev_len = 0;  // input at dev_off, output at dev_off + align(src)
+    size_t stage_off = 0;
+    bool staged = false;
+    bool done = false;
+    qtx_server_result res{};
+};
+using ReqP = std::shared_ptr<Req>;
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    BatchJob *d_jobs = nullptr;
+    BatchJob *h_jobs = nullptr;  // pinned staging of the job table
+    uint32_t *d_queues = nullptr;
+    std::vector<ReqP> reqs;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct qtx_server {
+    qtx_ctx *ctx = nullptr;
+    qtx_server_config cfg{};
+    uint8_t *d_arena = nullptr;
+    uint8_t *h_stage = nullptr;
+    srv::Arena arena, stage;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ReqP> waiting;
+    std::unordered_map<uint64_t, ReqP> tickets;
+    uint64_t next_ticket = 1;
+    bool stopping = false;
+    bool failed = false;
+    std::string fail_msg;
+    std::thread loop;
+    std::chrono::steady_clock::time_point origin;
+    qtx_server_stats stats{};
+    double busy_since = -1.0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - origin).count();
+    }
+};
+
+namespace {
+
+// Enqueue one batch on a slot (called without the lock; touches only the
+// slot and the requests, which the loop owns while they are admitted).
+hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
+    hipStream_t st = sl.stream;
+    std::vector<bt::Prepared> prep(reqs.size());
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
+        const Req &r = *reqs[i];
+        prep[i] = r.prep;
+        uint8_t *in = s->d_arena + r.dev_off;
+        prep[i].job.src = in;
+        prep[i].job.dst = in + align_up(r.prep.src_bytes);
+        e = hipMemcpyAsync(in, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, st);
+    }
+    if (e != hipSuccess) return e;
+    bt::Plan plan;
+    const qtx_batch_tuning tuning{bt::kMaxColourWgPerCu, s->cfg.unit_kops};
+    if (bt::plan(prep, tuning, s->ctx->cu_count, plan) != QTX_OK) return hipErrorInvalidValue;
+    std::memcpy(sl.h_jobs, plan.host.data(), plan.host.size() * sizeof(BatchJob));
+    e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st);
+    for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++)
+        e = bt::enqueue_group(plan.groups[g], sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
+    for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
+        const Req &r = *reqs[i];
+        void *dst = r.staged ? (void *)(s->h_stage + r.stage_off) : r.job.dst;
+        e = hipMemcpyAsync(dst, prep[i].job.dst, r.prep.dst_bytes, hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(sl.done, st);
+    return e;
+}
+
+void release(qtx_server *s, Req &r) {
+    s->arena.release(r.dev_off, r.dev_len);
+    if (r.staged) s->stage.release(r.stage_off, r.prep.dst_bytes);
+}
+
+void finish(qtx_server *s, Req &r, qtx_status st, const char *msg, double t) {
+    r.res.status = st;
+    if (msg) std::snprintf(r.res.message, sizeof r.res.message, "%s", msg);
+    r.res.done_s = t;
+    r.done = true;
+    if (st == QTX_OK) {
+        s->stats.completed++;
+        s->stats.pixels += (uint64_t)r.job.width * (uint64_t)r.job.height;
+    } else {
+        s->stats.failed++;
+    }
+}
+
+void update_busy(qtx_server *s, double t) {
+    const bool any = std::any_of(s->slots.begin(), s->slots.end(), [](const Slot &x) { return x.busy; });
+    if (any && s->busy_since < 0) s->busy_since = t;
+    if (!any && s->busy_since >= 0) {
+        s->stats.busy_s += t - s->busy_since;
+        s->busy_since = -1.0;
+    }
+}
+
+void run(qtx_server *s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        // 1. completed batches
+        for (Slot &sl : s->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) continue;
+            std::vector<ReqP> reqs;
+            reqs.swap(sl.reqs);
+            lk.unlock();
+            if (q == hipSuccess)
+                for (auto &r : reqs)
+                    if (r->staged) std::memcpy(r->job.dst, s->h_stage + r->stage_off, r->prep.dst_bytes);
+            lk.lock();
+            const double t = s->now();
+            std::string msg;
+            if (q != hipSuccess) {
+                msg = std::string("batch failed: ") + hipGetErrorString(q);
+                s->failed = true;
+                s->fail_msg = msg;
+            }
+            for (auto &r : reqs) {
+                release(s, *r);
+                finish(s, *r, q == hipSuccess ? QTX_OK : QTX_EDEVICE, q == hipSuccess ? nullptr : msg.c_str(), t);
+            }
+            sl.busy = false;
+            update_busy(s, t);
+            s->cv_done.notify_all();
+        }
+        // 2. a new batch on a free slot
+        Slot *free_slot = nullptr;
+        for (Slot &sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (free_slot && !s->waiting.empty() && !s->failed) {
+            std::deque<ReqP> taken;
+            auto reserve = [&](ReqP &r) {
+                size_t off = 0, soff = 0;
+                const size_t need = align_up(r->prep.src_bytes) + align_up(r->prep.dst_bytes);
+                if (!s->arena.alloc(need, off)) return false;
+                if (!r->dst_pinned) {
+                    if (!s->stage.alloc(r->prep.dst_bytes, soff)) {
+                        s->arena.release(off, need);
+                        return false;
+                    }
+                    r->stage_off = soff;
+                    r->staged = true;
+                }
+                r->dev_off = off;
+                r->dev_len = need;
+                return true;
+            };
+            srv::admit_fifo(s->waiting, (size_t)s->cfg.max_batch_jobs, reserve, taken);
+            if (!taken.empty()) {
+                std::vector<ReqP> reqs(taken.begin(), taken.end());
+                const double t_admit = s->now();
+                for (auto &r : reqs) {
+                    r->res.admit_s = t_admit;
+                    r->res.batch_jobs = (int32_t)reqs.size();
+                }
+                s->stats.batches++;
+                s->stats.max_batch_jobs = std::max<uint64_t>(s->stats.max_batch_jobs, reqs.size());
+                s->stats.device_bytes_peak = std::max(s->stats.device_bytes_peak, s->arena.used());
+                free_slot->busy = true;
+                update_busy(s, t_admit);
+                lk.unlock();
+                const hipError_t e = enqueue(s, *free_slot, reqs);
+                lk.lock();
+                const double t_launch = s->now();
+                if (e != hipSuccess) {
+                    // nothing of this batch can be trusted; wait for what was enqueued
+                    (void)hipStreamSynchronize(free_slot->stream);
+                    const std::string msg = std::string("batch enqueue failed: ") + hipGetErrorString(e);
+                    s->failed = true;
+                    s->fail_msg = msg;
+                    for (auto &r : reqs) {
+                        release(s, *r);
+                        finish(s, *r, QTX_EDEVICE, msg.c_str(), t_launch);
+                    }
+                    free_slot->busy = false;
+                    update_busy(s, t_launch);
+                    s->cv_done.notify_all();
+                } else {
+                    for (auto &r : reqs) r->res.launch_s = t_launch;
+                    free_slot->reqs = std::move(reqs);
+                }
+                continue;  // look for completions and another free slot at once
+            }
+        }
+        const bool inflight = std::any_of(s->slots.begin(), s->slots.end(), [](const Slot &x) { return x.busy; });
+        if (s->failed && !inflight) {
+            // fail everything still waiting
+            const double t = s->now();
+            for (auto &r : s->waiting) finish(s, *r, QTX_EDEVICE, s->fail_msg.c_str(), t);
+            s->waiting.clear();
+            s->cv_done.notify_all();
+        }
+        if (s->stopping && s->waiting.empty() && !inflight) break;
+        if (inflight)
+            s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+        else
+            s->cv_work.wait(lk, [&] { return s->stopping || (!s->waiting.empty() && !s->failed); });
+    }
+}
+
+void destroy(qtx_server *s) {
+    (void)hipSetDevice(s->ctx->device);
+    for (Slot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.d_jobs) (void)hipFree(sl.d_jobs);
+        if (sl.d_queues) (void)hipFree(sl.d_queues);
+        if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
+    }
+    if (s->d_arena) (void)hipFree(s->d_arena);
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_server **out) {
+    if (!out) return fail(QTX_EINVAL, "null output pointer");
+    *out = nullptr;
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    qtx_server_config c{};
+    if (config) c = *config;
+    if (!c.device_bytes) c.device_bytes = (size_t)2 << 30;
+    if (!c.staging_bytes) c.staging_bytes = (size_t)256 << 20;
+    if (!c.slots) c.slots = 3;
+    if (!c.max_batch_jobs) c.max_batch_jobs = 1024;
+    if (!c.max_queue) c.max_queue = 65536;
+    if (!c.unit_kops) c.unit_kops = 64;
+    if (c.slots < 1 || c.slots > 8) return fail(QTX_EINVAL, "slots must be in 1..8");
+    if (c.max_batch_jobs < 1 || c.max_batch_jobs > 65536) return fail(QTX_EINVAL, "max_batch_jobs must be in 1..65536");
+    if (c.max_queue < 1) return fail(QTX_EINVAL, "max_queue must be positive");
+    if (c.unit_kops < 16 || c.unit_kops > 16384) return fail(QTX_EINVAL, "unit_kops must be in 16..16384");
+    if (c.device_bytes < (1u << 20)) return fail(QTX_EINVAL, "device_bytes must be at least 1 MiB");
+
+    auto *s = new qtx_server;
+    s->ctx = ctx;
+    s->cfg = c;
+    s->origin = std::chrono::steady_clock::now();
+    hipError_t e = hipMalloc(&s->d_arena, c.device_bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_stage, c.staging_bytes, hipHostMallocDefault);
+    s->slots.resize((size_t)c.slots);
+    for (Slot &sl : s->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob));
+        if (e == hipSuccess)
+            e = hipHostMalloc((void **)&sl.h_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_queues, kQueueWords * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(sl.d_queues, 0, kQueueWords * sizeof(uint32_t));
+    }
+    if (e != hipSuccess) {
+        const qtx_status r = e == hipErrorOutOfMemory ? fail(QTX_ENOMEM, "server set-up: %s", hipGetErrorString(e))
+                                                      : hip_fail(e, "server set-up");
+        destroy(s);
+        return r;
+    }
+    s->arena.reset(c.device_bytes, kAlign);
+    s->stage.reset(c.staging_bytes, kAlign);
+    s->loop = std::thread(run, s);
+    *out = s;
+    return QTX_OK;
+}
+
+qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes, uint64_t *ticket) {
+    if (!s || !job || !ticket) return fail(QTX_EINVAL, "null argument");
+    auto r = std::make_shared<Req>();
+    r->job = *job;
+    qtx_status st = bt::prepare(*job, bt::kNoIndex, r->prep);
+    if (st != QTX_OK) return st;
+    if (dst_bytes < r->prep.dst_bytes)
+        return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", dst_bytes, r->prep.dst_bytes);
+    const size_t need = align_up(r->prep.src_bytes) + align_up(r->prep.dst_bytes);
+    if (need > s->arena.capacity())
+        return fail(QTX_ENOMEM, "request needs %zu bytes of device memory; the server has %zu", need,
+                    s->arena.capacity());
+    (void)hipSetDevice(s->ctx->device);
+    r->dst_pinned = is_pinned(job->dst);
+    if (!r->dst_pinned && align_up(r->prep.dst_bytes) > s->stage.capacity())
+        return fail(QTX_ENOMEM, "output of %zu bytes to pageable memory exceeds the %zu-byte staging area",
+                    r->prep.dst_bytes, s->stage.capacity());
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return fail(QTX_EINVAL, "server is stopping");
+    if (s->failed) return fail(QTX_EDEVICE, "server failed: %s", s->fail_msg.c_str());
+    if (s->waiting.size() >= (size_t)s->cfg.max_queue)
+        return fail(QTX_EBUSY, "queue full (%d requests waiting)", s->cfg.max_queue);
+    r->ticket = s->next_ticket++;
+    r->res.submit_s = s->now();
+    s->waiting.push_back(r);
+    s->tickets[r->ticket] = r;
+    s->stats.submitted++;
+    s->stats.max_waiting = std::max<uint64_t>(s->stats.max_waiting, s->waiting.size());
+    *ticket = r->ticket;
+    s->cv_work.notify_one();
+    return QTX_OK;
+}
+
+qtx_status qtx_server_wait(qtx_server *s, uint64_t ticket, int32_t timeout_ms, qtx_server_result *result) {
+    if (!s || !result) return fail(QTX_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(QTX_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+    ReqP r = it->second;
+    if (timeout_ms < 0)
+        s->cv_done.wait(lk, [&] { return r->done; });
+    else
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return r->done; });
+    if (!r->done) return fail(QTX_EBUSY, "request %llu not finished", (unsigned long long)ticket);
+    *result = r->res;
+    s->tickets.erase(ticket);
+    return QTX_OK;
+}
+
+qtx_status qtx_server_stats_get(qtx_server *s, qtx_server_stats *stats) {
+    if (!s || !stats) return fail(QTX_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *stats = s->stats;
+    if (s->busy_since >= 0) stats->busy_s += s->now() - s->busy_since;
+    return QTX_OK;
+}
+
+double qtx_server_now(qtx_server *s) { return s ? s->now() : 0.0; }
+
+qtx_status qtx_server_stop(qtx_server *s) {
+    if (!s) return QTX_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    if (s->loop.joinable()) s->loop.join();
+    destroy(s);
+    return QTX_OK;
+}
+
+}  // extern "C"

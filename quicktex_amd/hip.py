@@ -122,6 +122,29 @@ class Buffer:
             pass
 
 
+class Pinned:
+    """nbytes of pinned (page-locked) host memory; .array is a uint8 numpy view."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().hipHostMalloc(ctypes.byref(p), max(self.nbytes, 1), 0), 'hipHostMalloc(%d)' % self.nbytes)
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(self.nbytes, 1)).from_address(self.ptr))[:self.nbytes]
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().hipHostFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     """A non-blocking stream at the default priority."""
 
