@@ -25,6 +25,14 @@ mixture, alpha smoothed noise with 15% binary blocks -- BC3 level 10 encoded
 (a) with one plain launch per mip and (b) as one batch (HIP graph, autotuned);
 outputs must be byte-equal, and a sample is checked against the CPU.
 
+served: 2000 requests -- BC1/BC3/BC4/BC5 at 40/30/15/15%, encode:decode 50:50,
+sides log-uniform 32..4096, levels 0..18; encode sources of the block mixture
+(BC3 with noise alpha) and smoothed channels (BC4/BC5), decode sources their
+level-5 encodes -- from pinned host buffers: (a) each request alone through the
+plain host entry point, one after another; (b) all submitted at once to the
+serving loop; (c) submitted at Poisson arrivals of a fixed rate, for p50/p99
+latency.  Outputs of a sample of requests must be byte-equal across (a)-(c).
+
 Prints one JSON line.
 """
 import argparse
@@ -265,6 +273,242 @@ def batched_leg(a, q, hip, np):
 
 
 # ---------------------------------------------------------------------------
+def served_leg(a, q, hip, np):
+    import ctypes
+
+    from quicktex_amd import synth
+    from quicktex_amd.capi import (COLOR_FOUR, FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5, OP_DECODE, OP_ENCODE, Server,
+                                   job)
+
+    N = a.serve_requests
+    rng = np.random.default_rng(0x5EED0004)
+    S = 4096
+    log('serving workload: %d requests' % N)
+    # request sources live in pinned host memory, as a serving front end's receive buffers would
+    mix = synth.mixture_rgba(S, S, seed=0x5EED0000)
+    mix_a = mix.copy()
+    mix_a[..., 3] = synth.alpha_channel(S, S, 0x5EED0005)
+    smooth = synth.smooth_channels(S, S, seed=0x5EED0001, nchan=2)
+    masters = {}
+    for name, img in (('bc1', mix), ('bc3', mix_a), ('bc45', smooth)):
+        pm = hip.Pinned(img.nbytes)
+        pm.array[:] = img.reshape(-1)
+        masters[name] = pm
+    # decode sources: encodes of the same content (level 5)
+    d_img = hip.Buffer(S * S * 4)
+    blocks = {}
+    p5 = q.params(5, COLOR_FOUR, 0)
+    nbm = (S // 4) * (S // 4)
+    for fmt, src, bpb in ((FMT_BC1, 'bc1', 8), (FMT_BC3, 'bc3', 16), (FMT_BC4, 'bc45', 8), (FMT_BC5, 'bc45', 16)):
+        d_img.upload(masters[src].array)
+        d_blk = hip.Buffer(nbm * bpb)
+        if fmt == FMT_BC1:
+            q.bc1_encode_device(p5, d_img.ptr, S, S, d_blk.ptr, 0)
+        elif fmt == FMT_BC3:
+            q.bc3_encode_device(p5, d_img.ptr, S, S, d_blk.ptr, 0)
+        elif fmt == FMT_BC4:
+            q.bc4_encode_device(0, d_img.ptr, S, S, d_blk.ptr, 0)
+        else:
+            q.bc5_encode_device(0, 1, d_img.ptr, S, S, d_blk.ptr, 0)
+        hip.synchronize()
+        pb = hip.Pinned(nbm * bpb)
+        pb.array[:] = d_blk.download()
+        blocks[fmt] = pb
+    d_img.free()
+
+    reqs = []
+    out_total = 0
+    for i in range(N):
+        fmt = int(rng.choice([FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5], p=[0.4, 0.3, 0.15, 0.15]))
+        op = OP_ENCODE if rng.random() < 0.5 else OP_DECODE
+        w = int(32 * 128.0 ** rng.random())
+        h = int(32 * 128.0 ** rng.random())
+        level = int(rng.integers(0, 19))
+        bpb = 8 if fmt in (FMT_BC1, FMT_BC4) else 16
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        if op == OP_ENCODE:
+            src = masters['bc1' if fmt == FMT_BC1 else 'bc3' if fmt == FMT_BC3 else 'bc45'].array[:w * h * 4]
+            out_n = nb * bpb
+        else:
+            src = blocks[fmt].array[:nb * bpb]
+            out_n = w * h * 4
+        reqs.append(dict(fmt=fmt, op=op, w=w, h=h, level=level, src=src, out_off=out_total, out_n=out_n,
+                         params=q.params(level, COLOR_FOUR, 0)))
+        out_total += (out_n + 255) & ~255
+    outbuf = hip.Pinned(out_total)
+    npx = sum(r['w'] * r['h'] for r in reqs)
+    for r in reqs:
+        r['dst'] = outbuf.array[r['out_off']:r['out_off'] + r['out_n']]
+    log('  %.1f Mpixel in %d requests, %.2f GB of outputs' % (npx / 1e6, N, out_total / 1e9))
+
+    L = q.lib
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+
+    def plain(r):
+        s, d = r['src'].ctypes.data_as(u8p), r['dst'].ctypes.data_as(u8p)
+        w, h, n = r['w'], r['h'], r['out_n']
+        if r['op'] == OP_ENCODE:
+            if r['fmt'] == FMT_BC1:
+                st = L.qtx_bc1_encode(q.ctx, ctypes.byref(r['params']), s, w, h, d, n)
+            elif r['fmt'] == FMT_BC3:
+                st = L.qtx_bc3_encode(q.ctx, ctypes.byref(r['params']), s, w, h, d, n)
+            elif r['fmt'] == FMT_BC4:
+                st = L.qtx_bc4_encode(q.ctx, 0, s, w, h, d, n)
+            else:
+                st = L.qtx_bc5_encode(q.ctx, 0, 1, s, w, h, d, n)
+        else:
+            if r['fmt'] == FMT_BC1:
+                st = L.qtx_bc1_decode(q.ctx, 0, 0, s, w, h, d, n)
+            elif r['fmt'] == FMT_BC3:
+                st = L.qtx_bc3_decode(q.ctx, 0, s, w, h, d, n)
+            elif r['fmt'] == FMT_BC4:
+                st = L.qtx_bc4_decode(q.ctx, 0, s, w, h, d, n)
+            else:
+                st = L.qtx_bc5_decode(q.ctx, 0, 1, s, w, h, d, n)
+        q.check(st)
+
+    # plain: each request alone through the host entry point, one after another
+    for r in reqs[:8]:
+        plain(r)  # warm-up
+    service = np.zeros(N)
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        t = time.perf_counter()
+        plain(r)
+        service[i] = time.perf_counter() - t
+    plain_s = time.perf_counter() - t0
+    sample = list(range(0, N, 20))
+    plain_out = {i: reqs[i]['dst'].copy() for i in sample}
+    log('plain: %.3f s, %.1f Mpx/s' % (plain_s, npx / plain_s / 1e6))
+
+    def job_of(r):
+        return job(r['op'], r['fmt'], r['w'], r['h'], 0, 0, params=r['params'], chan0=0, chan1=1,
+                   interpolator=0)
+
+    def serve(arrivals):
+        outbuf.array[:] = 0
+        srv = Server(q, device_bytes=a.serve_device_mib << 20, staging_bytes=64 << 20, slots=3,
+                     max_batch_jobs=1024)
+        try:
+            jobs = [job_of(r) for r in reqs]
+            base = time.perf_counter()
+            tickets = []
+            for i, r in enumerate(reqs):
+                if arrivals is not None:
+                    while time.perf_counter() - base < arrivals[i]:
+                        pass
+                tickets.append(srv.submit(jobs[i], r['src'], r['dst']))
+            res = [srv.wait(t, 600000) for t in tickets]
+            stats = srv.stats()
+        finally:
+            srv.stop()
+        bad = [i for i, x in enumerate(res) if x.status != 0]
+        assert not bad, 'served requests failed: %s' % bad[:5]
+        sub = np.array([x.submit_s for x in res])
+        done = np.array([x.done_s for x in res])
+        span = done.max() - sub.min()
+        lat = (done - sub) * 1000.0
+        exact = all(np.array_equal(plain_out[i], reqs[i]['dst']) for i in sample)
+        return span, lat, stats, exact, np.array([x.batch_jobs for x in res])
+
+    sat_span, sat_lat, sat_stats, sat_exact, sat_bj = serve(None)
+    sat_rate = N / sat_span
+    log('served (all queued at once): %.3f s, %.1f Mpx/s, %d batches' % (sat_span, npx / sat_span / 1e6,
+                                                                        sat_stats['batches']))
+    lam = a.serve_rate if a.serve_rate > 0 else 0.6 * sat_rate
+    gaps = rng.exponential(1.0 / lam, N)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    po_span, po_lat, po_stats, po_exact, po_bj = serve(arrivals)
+    log('served (Poisson %.0f req/s): %.3f s, p50 %.2f ms, p99 %.2f ms' % (
+        lam, po_span, np.percentile(po_lat, 50), np.percentile(po_lat, 99)))
+    # the plain path under the same arrivals: one request at a time in arrival order
+    fin = 0.0
+    plain_lat = np.zeros(N)
+    for i in range(N):
+        fin = max(fin, arrivals[i]) + service[i]
+        plain_lat[i] = (fin - arrivals[i]) * 1000.0
+    for pm in list(masters.values()) + list(blocks.values()) + [outbuf]:
+        pm.free()
+    pct = lambda x, p: round(float(np.percentile(x, p)), 3)  # noqa: E731
+    return {
+        'workload': ('%d requests: BC1/BC3/BC4/BC5 40/30/15/15%%, encode:decode 50:50, sides log-uniform 32..4096, '
+                     'levels 0..18 (FourColor, Ideal), %.1f Mpixel; host buffers pinned' % (N, npx / 1e6)),
+        'plain_mpx_s': round(npx / plain_s / 1e6, 1),
+        'served_mpx_s': round(npx / sat_span / 1e6, 1),
+        'served_vs_plain': round(plain_s / sat_span, 2),
+        'served_batches': sat_stats['batches'], 'served_mean_batch': round(float(sat_bj.mean()), 1),
+        'poisson': {'rate_req_s': round(lam, 1), 'requests': N,
+                    'throughput_mpx_s': round(npx / po_span / 1e6, 1),
+                    'p50_ms': pct(po_lat, 50), 'p99_ms': pct(po_lat, 99), 'mean_ms': round(float(po_lat.mean()), 3),
+                    'batches': po_stats['batches'], 'mean_batch': round(float(po_bj.mean()), 1),
+                    'plain_same_arrivals_p50_ms': pct(plain_lat, 50),
+                    'plain_same_arrivals_p99_ms': pct(plain_lat, 99)},
+        'device_bytes_peak': int(max(sat_stats['device_bytes_peak'], po_stats['device_bytes_peak'])),
+        'device_budget_bytes': a.serve_device_mib << 20,
+        'bit_exact_served_vs_plain_sample': bool(sat_exact and po_exact),
+    }
+
+
+# ---------------------------------------------------------------------------
+def sweep_leg(a, q, hip, np):
+    """Plain-kernel throughput per BC1 level (FourColor, Ideal) and of the
+    HBM-bound kernels (BC4/BC5 encode, every decoder) on 4096x4096 inputs."""
+    from quicktex_amd import synth
+    from quicktex_amd.capi import COLOR_FOUR
+    from tests._cpu_libs import op_count
+
+    S = a.sweep_size
+    img = synth.mixture_rgba(S, S, seed=0x5EED0002, near_black=0.10)
+    sm = synth.smooth_channels(S, S, seed=0x5EED0001, nchan=2)
+    nb = (S // 4) * (S // 4)
+    d_img = hip.Buffer.from_array(img)
+    d_sm = hip.Buffer.from_array(sm)
+    d_blk = hip.Buffer(nb * 16)
+    d_out = hip.Buffer(S * S * 4)
+    stream = hip.Stream()
+    sh = stream.handle
+    sample = np.ascontiguousarray(img[:16])
+
+    def timeit(fn, reps=3):
+        fn()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    levels = []
+    for lvl in range(20):
+        p = q.params(lvl, COLOR_FOUR, 0)
+        ms = timeit(lambda: q.bc1_encode_device(p, d_img.ptr, S, S, d_blk.ptr, sh))
+        counts, weights = op_count(lambda o: o.bc1_encode(sample, p))
+        ops = sum(c * w for c, w in zip(counts, weights)) / ((S // 4) * 4)
+        tops = ops * nb / (ms / 1e3) / 1e12
+        levels.append({'level': lvl, 'ms': round(ms, 4), 'mpx_s': round(S * S / ms / 1e3, 1),
+                       'ops_per_block': round(ops), 'valu_frac': round(tops / VALU_PEAK_TOPS, 4),
+                       'hbm_frac': round((S * S * 4.5) / (ms / 1e3) / 8e12, 4)})
+        log('level %2d: %.3f ms, %.0f Mpx/s, %.0f ops/block, VALU %.1f%%' % (
+            lvl, ms, S * S / ms / 1e3, ops, 100 * tops / VALU_PEAK_TOPS))
+    hbm = {}
+    cases = [('bc4_encode', 4.5, lambda: q.bc4_encode_device(0, d_sm.ptr, S, S, d_blk.ptr, sh)),
+             ('bc5_encode', 5.0, lambda: q.bc5_encode_device(0, 1, d_sm.ptr, S, S, d_blk.ptr, sh)),
+             ('bc1_decode', 4.5, lambda: q.bc1_decode_device(0, 0, d_blk.ptr, S, S, d_out.ptr, sh)),
+             ('bc3_decode', 5.0, lambda: q.bc3_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
+             ('bc4_decode', 4.5, lambda: q.bc4_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
+             ('bc5_decode', 5.0, lambda: q.bc5_decode_device(0, 1, d_blk.ptr, S, S, d_out.ptr, sh))]
+    for name, bpp, fn in cases:
+        ms = timeit(fn, 10)
+        gbs = S * S * bpp / (ms / 1e3) / 1e9
+        hbm[name] = {'ms': round(ms, 4), 'mpx_s': round(S * S / ms / 1e3, 1), 'gb_s': round(gbs, 1),
+                     'hbm_frac': round(gbs / 8000.0, 4)}
+        log('%s: %.4f ms, %.0f GB/s (%.1f%% of 8 TB/s)' % (name, ms, gbs, gbs / 80.0))
+    stream.destroy()
+    return {'size': '%dx%d' % (S, S), 'bc1_levels_fourcolor_ideal': levels, 'hbm_bound': hbm}
+
+
+# ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -272,7 +516,12 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--legs', default='plain,batched', help='comma list of: plain, batched')
+    ap.add_argument('--legs', default='plain,batched,served', help='comma list of: plain, batched, served, sweep')
+    ap.add_argument('--serve-requests', type=int, default=2000)
+    ap.add_argument('--serve-rate', type=float, default=0.0,
+                    help='Poisson arrival rate, requests/s (default: 60%% of the measured saturation rate)')
+    ap.add_argument('--serve-device-mib', type=int, default=4096, help='device budget of the server')
+    ap.add_argument('--sweep-size', type=int, default=4096, help='texture side of the sweep leg')
     ap.add_argument('--batch-textures', type=int, default=300)
     ap.add_argument('--batch-steps', type=int, default=5)
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
@@ -316,6 +565,10 @@ def main():
             line['gpu_vs_cpu'] = round(value / cpu['value'], 1)
     if 'batched' in legs:
         line['batched'] = batched_leg(a, q, hip, np)
+    if 'served' in legs:
+        line['served'] = served_leg(a, q, hip, np)
+    if 'sweep' in legs:
+        line['sweep'] = sweep_leg(a, q, hip, np)
     print(json.dumps(line), flush=True)
     q.close()
 

@@ -224,6 +224,9 @@ typedef struct qtx_server_config {
     int32_t max_batch_jobs; /* requests per batch, 1..65536 (0: 1024)                     */
     int32_t max_queue;      /* waiting requests before submit returns EBUSY (0: 65536)    */
     uint32_t unit_kops;     /* colour work-unit size, as in qtx_batch_tuning; 0: 64        */
+    size_t batch_bytes;     /* bytes a batch may move (device memory + the PCIe traffic of
+                               kernels that read or write host memory); a larger
+                               request runs alone (0: min(device_bytes / slots, 256 MiB)) */
 } qtx_server_config;
 
 typedef struct qtx_server_result {

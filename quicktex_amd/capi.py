@@ -49,7 +49,8 @@ class BatchTuning(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     """qtx_server_config (include/qtx.h); zero fields take the defaults."""
     _fields_ = [('device_bytes', ctypes.c_size_t), ('staging_bytes', ctypes.c_size_t), ('slots', ctypes.c_int32),
-                ('max_batch_jobs', ctypes.c_int32), ('max_queue', ctypes.c_int32), ('unit_kops', ctypes.c_uint32)]
+                ('max_batch_jobs', ctypes.c_int32), ('max_queue', ctypes.c_int32), ('unit_kops', ctypes.c_uint32),
+                ('batch_bytes', ctypes.c_size_t)]
 
 
 class ServerResult(ctypes.Structure):
@@ -299,9 +300,10 @@ class Batch:
 class Server:
     """A qtx_server: submit host-memory requests, wait for their results."""
 
-    def __init__(self, q, device_bytes=0, staging_bytes=0, slots=0, max_batch_jobs=0, max_queue=0, unit_kops=0):
+    def __init__(self, q, device_bytes=0, staging_bytes=0, slots=0, max_batch_jobs=0, max_queue=0, unit_kops=0,
+                 batch_bytes=0):
         self.q = q
-        cfg = ServerConfig(device_bytes, staging_bytes, slots, max_batch_jobs, max_queue, unit_kops)
+        cfg = ServerConfig(device_bytes, staging_bytes, slots, max_batch_jobs, max_queue, unit_kops, batch_bytes)
         h = ctypes.c_void_p()
         q.check(q.lib.qtx_server_start(q.ctx, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -5,12 +5,15 @@
 // whose input + output fit in the device arena (and, for a destination in
 // pageable memory, in the pinned staging area), stopping at the first that does
 // not fit so that no request is overtaken forever.  The admitted requests form
-// one batch, enqueued on the slot's stream:
-//     H2D of the inputs -> job table -> the batch kernels -> D2H of the outputs
+// one batch (at most batch_bytes of device memory, so that several batches fit),
+// enqueued on the slot's stream:
+//     H2D of the inputs -> job table -> the batch kernels -> D2H of the outputs,
+//     smallest output first, an event after each
 // and the loop goes straight back to admitting: up to `slots` batches are on
 // the device at once, so one batch's copies overlap another's kernels and
-// requests that arrive while a batch runs join the next one.  A slot whose
-// completion event has fired returns its memory and completes its requests.
+// requests that arrive while a batch runs join the next one.  A request whose
+// event has fired is complete and returns its memory; a slot is free again when
+// all of its requests are.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,9 +61,11 @@ struct Req {
     qtx_job job{};  // host pointers
     bt::Prepared prep{};
     bool dst_pinned = false;
-    size_t dev_off = 0, dev_len = 0;  // input at dev_off, output at dev_off + align(src)
+    bool src_direct = false;  // the kernel reads the caller's pinned input in place
+    bool out_direct = false;  // the kernel writes the output into host memory (decodes)
+    size_t dev_off = 0, dev_len = 0;  // device input (unless src_direct), then device output (unless out_direct)
     size_t stage_off = 0;
-    bool staged = false;
+    bool staged = false;  // the output goes through pinned staging (pageable destination)
     bool done = false;
     qtx_server_result res{};
 };
@@ -68,11 +73,12 @@ using ReqP = std::shared_ptr<Req>;
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> events;  // events[i]: request i's output is in its destination
     BatchJob *d_jobs = nullptr;
     BatchJob *h_jobs = nullptr;  // pinned staging of the job table
     uint32_t *d_queues = nullptr;
-    std::vector<ReqP> reqs;
+    std::vector<ReqP> reqs;  // in output-copy order
+    size_t next_done = 0;    // requests before this one are complete
     bool busy = false;
 };
 
@@ -105,8 +111,21 @@ struct qtx_server {
 
 namespace {
 
+// Device memory a request needs: its input unless the kernel reads it in place,
+// its output unless the kernel writes it to host memory.
+size_t footprint(const Req &r) {
+    return (r.src_direct ? 0 : align_up(r.prep.src_bytes)) + (r.out_direct ? 0 : align_up(r.prep.dst_bytes));
+}
+
 // Enqueue one batch on a slot (called without the lock; touches only the
-// slot and the requests, which the loop owns while they are admitted).
+// slot and the requests, which the loop owns while they are admitted).  reqs
+// holds the decodes first, then the encodes by ascending output size.
+//   decodes: blocks are read from the caller's pinned buffer in place (or
+//     copied in when pageable) and the kernel stores texels straight into host
+//     memory -- the caller's pinned buffer or the staging area -- so the large
+//     direction of a decode never queues behind the copy engine;
+//   encodes: input copied in, blocks copied out, smallest output first.
+// events[i] fires when request i's output is in host memory.
 hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
     hipStream_t st = sl.stream;
     std::vector<bt::Prepared> prep(reqs.size());
@@ -114,10 +133,18 @@ hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
     for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
         const Req &r = *reqs[i];
         prep[i] = r.prep;
-        uint8_t *in = s->d_arena + r.dev_off;
-        prep[i].job.src = in;
-        prep[i].job.dst = in + align_up(r.prep.src_bytes);
-        e = hipMemcpyAsync(in, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, st);
+        uint8_t *dev = s->d_arena + r.dev_off;
+        const size_t in_len = r.src_direct ? 0 : align_up(r.prep.src_bytes);
+        if (r.src_direct) {
+            prep[i].job.src = static_cast<const uint8_t *>(r.job.src);
+        } else {
+            prep[i].job.src = dev;
+            e = hipMemcpyAsync(dev, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, st);
+        }
+        if (r.out_direct)
+            prep[i].job.dst = r.staged ? s->h_stage + r.stage_off : static_cast<uint8_t *>(r.job.dst);
+        else
+            prep[i].job.dst = dev + in_len;
     }
     if (e != hipSuccess) return e;
     bt::Plan plan;
@@ -129,15 +156,17 @@ hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
         e = bt::enqueue_group(plan.groups[g], sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
     for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
         const Req &r = *reqs[i];
-        void *dst = r.staged ? (void *)(s->h_stage + r.stage_off) : r.job.dst;
-        e = hipMemcpyAsync(dst, prep[i].job.dst, r.prep.dst_bytes, hipMemcpyDeviceToHost, st);
+        if (!r.out_direct) {
+            void *dst = r.staged ? (void *)(s->h_stage + r.stage_off) : r.job.dst;
+            e = hipMemcpyAsync(dst, prep[i].job.dst, r.prep.dst_bytes, hipMemcpyDeviceToHost, st);
+        }
+        if (e == hipSuccess) e = hipEventRecord(sl.events[i], st);
     }
-    if (e == hipSuccess) e = hipEventRecord(sl.done, st);
     return e;
 }
 
 void release(qtx_server *s, Req &r) {
-    s->arena.release(r.dev_off, r.dev_len);
+    if (r.dev_len) s->arena.release(r.dev_off, r.dev_len);
     if (r.staged) s->stage.release(r.stage_off, r.prep.dst_bytes);
 }
 
@@ -167,32 +196,46 @@ void run(qtx_server *s) {
     (void)hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        // 1. completed batches
+        // 1. completed requests, in each slot's output-copy order
         for (Slot &sl : s->slots) {
             if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.done);
-            if (q == hipErrorNotReady) continue;
-            std::vector<ReqP> reqs;
-            reqs.swap(sl.reqs);
-            lk.unlock();
-            if (q == hipSuccess)
-                for (auto &r : reqs)
-                    if (r->staged) std::memcpy(r->job.dst, s->h_stage + r->stage_off, r->prep.dst_bytes);
-            lk.lock();
-            const double t = s->now();
-            std::string msg;
-            if (q != hipSuccess) {
-                msg = std::string("batch failed: ") + hipGetErrorString(q);
-                s->failed = true;
-                s->fail_msg = msg;
-            }
-            for (auto &r : reqs) {
+            const size_t done_before = sl.next_done;
+            while (sl.next_done < sl.reqs.size()) {
+                const hipError_t q = hipEventQuery(sl.events[sl.next_done]);
+                if (q == hipErrorNotReady) break;
+                if (q != hipSuccess) {
+                    const std::string msg = std::string("batch failed: ") + hipGetErrorString(q);
+                    s->failed = true;
+                    s->fail_msg = msg;
+                    lk.unlock();
+                    (void)hipStreamSynchronize(sl.stream);
+                    lk.lock();
+                    const double t = s->now();
+                    for (size_t i = sl.next_done; i < sl.reqs.size(); i++) {
+                        release(s, *sl.reqs[i]);
+                        finish(s, *sl.reqs[i], QTX_EDEVICE, msg.c_str(), t);
+                    }
+                    sl.next_done = sl.reqs.size();
+                    break;
+                }
+                ReqP r = sl.reqs[sl.next_done];
+                if (r->staged) {
+                    lk.unlock();
+                    std::memcpy(r->job.dst, s->h_stage + r->stage_off, r->prep.dst_bytes);
+                    lk.lock();
+                }
                 release(s, *r);
-                finish(s, *r, q == hipSuccess ? QTX_OK : QTX_EDEVICE, q == hipSuccess ? nullptr : msg.c_str(), t);
+                finish(s, *r, QTX_OK, nullptr, s->now());
+                sl.next_done++;
             }
-            sl.busy = false;
-            update_busy(s, t);
-            s->cv_done.notify_all();
+            const bool progressed = sl.next_done != done_before;
+            if (sl.next_done == sl.reqs.size()) {
+                sl.reqs.clear();
+                sl.next_done = 0;
+                sl.busy = false;
+                update_busy(s, s->now());
+            }
+            if (progressed) s->cv_done.notify_all();
         }
         // 2. a new batch on a free slot
         Slot *free_slot = nullptr;
@@ -203,13 +246,19 @@ void run(qtx_server *s) {
             }
         if (free_slot && !s->waiting.empty() && !s->failed) {
             std::deque<ReqP> taken;
+            size_t batch_host_bytes = 0;
             auto reserve = [&](ReqP &r) {
                 size_t off = 0, soff = 0;
-                const size_t need = align_up(r->prep.src_bytes) + align_up(r->prep.dst_bytes);
-                if (!s->arena.alloc(need, off)) return false;
+                const size_t need = footprint(*r);
+                // a batch holds at most batch_bytes (a larger request goes alone), so
+                // several batches fit in the arena and their copies and kernels overlap;
+                // bytes the kernels move over PCIe count against the same limit
+                const size_t moved = need + (r->out_direct ? r->prep.dst_bytes : 0) + (r->src_direct ? r->prep.src_bytes : 0);
+                if (batch_host_bytes > 0 && batch_host_bytes + moved > s->cfg.batch_bytes) return false;
+                if (need && !s->arena.alloc(need, off)) return false;
                 if (!r->dst_pinned) {
                     if (!s->stage.alloc(r->prep.dst_bytes, soff)) {
-                        s->arena.release(off, need);
+                        if (need) s->arena.release(off, need);
                         return false;
                     }
                     r->stage_off = soff;
@@ -217,11 +266,21 @@ void run(qtx_server *s) {
                 }
                 r->dev_off = off;
                 r->dev_len = need;
+                batch_host_bytes += moved;
                 return true;
             };
             srv::admit_fifo(s->waiting, (size_t)s->cfg.max_batch_jobs, reserve, taken);
             if (!taken.empty()) {
                 std::vector<ReqP> reqs(taken.begin(), taken.end());
+                std::stable_sort(reqs.begin(), reqs.end(), [](const ReqP &x, const ReqP &y) {
+                    if (x->out_direct != y->out_direct) return x->out_direct;  // decodes first
+                    return x->prep.dst_bytes < y->prep.dst_bytes;
+                });
+                while (free_slot->events.size() < reqs.size()) {
+                    hipEvent_t ev = nullptr;
+                    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) break;
+                    free_slot->events.push_back(ev);
+                }
                 const double t_admit = s->now();
                 for (auto &r : reqs) {
                     r->res.admit_s = t_admit;
@@ -233,7 +292,8 @@ void run(qtx_server *s) {
                 free_slot->busy = true;
                 update_busy(s, t_admit);
                 lk.unlock();
-                const hipError_t e = enqueue(s, *free_slot, reqs);
+                const hipError_t e = free_slot->events.size() < reqs.size() ? hipErrorOutOfMemory
+                                                                            : enqueue(s, *free_slot, reqs);
                 lk.lock();
                 const double t_launch = s->now();
                 if (e != hipSuccess) {
@@ -252,6 +312,7 @@ void run(qtx_server *s) {
                 } else {
                     for (auto &r : reqs) r->res.launch_s = t_launch;
                     free_slot->reqs = std::move(reqs);
+                    free_slot->next_done = 0;
                 }
                 continue;  // look for completions and another free slot at once
             }
@@ -276,7 +337,7 @@ void destroy(qtx_server *s) {
     (void)hipSetDevice(s->ctx->device);
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.done) (void)hipEventDestroy(sl.done);
+        for (hipEvent_t ev : sl.events) (void)hipEventDestroy(ev);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.d_jobs) (void)hipFree(sl.d_jobs);
         if (sl.d_queues) (void)hipFree(sl.d_queues);
@@ -304,6 +365,7 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     if (!c.max_batch_jobs) c.max_batch_jobs = 1024;
     if (!c.max_queue) c.max_queue = 65536;
     if (!c.unit_kops) c.unit_kops = 64;
+    if (!c.batch_bytes) c.batch_bytes = std::min(c.device_bytes / (size_t)c.slots, (size_t)256 << 20);
     if (c.slots < 1 || c.slots > 8) return fail(QTX_EINVAL, "slots must be in 1..8");
     if (c.max_batch_jobs < 1 || c.max_batch_jobs > 65536) return fail(QTX_EINVAL, "max_batch_jobs must be in 1..65536");
     if (c.max_queue < 1) return fail(QTX_EINVAL, "max_queue must be positive");
@@ -319,7 +381,6 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     s->slots.resize((size_t)c.slots);
     for (Slot &sl : s->slots) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob));
         if (e == hipSuccess)
             e = hipHostMalloc((void **)&sl.h_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob), hipHostMallocDefault);
@@ -347,12 +408,16 @@ qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes
     if (st != QTX_OK) return st;
     if (dst_bytes < r->prep.dst_bytes)
         return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", dst_bytes, r->prep.dst_bytes);
-    const size_t need = align_up(r->prep.src_bytes) + align_up(r->prep.dst_bytes);
+    (void)hipSetDevice(s->ctx->device);
+    r->dst_pinned = is_pinned(job->dst);
+    if (job->op == QTX_OP_DECODE) {
+        r->out_direct = true;
+        r->src_direct = is_pinned(job->src);
+    }
+    const size_t need = footprint(*r);
     if (need > s->arena.capacity())
         return fail(QTX_ENOMEM, "request needs %zu bytes of device memory; the server has %zu", need,
                     s->arena.capacity());
-    (void)hipSetDevice(s->ctx->device);
-    r->dst_pinned = is_pinned(job->dst);
     if (!r->dst_pinned && align_up(r->prep.dst_bytes) > s->stage.capacity())
         return fail(QTX_ENOMEM, "output of %zu bytes to pageable memory exceeds the %zu-byte staging area",
                     r->prep.dst_bytes, s->stage.capacity());
