@@ -2,10 +2,12 @@
 //
 // A batch is a fixed list of jobs (one texture each, any size, format and
 // encoder settings) launched as one HIP graph:
-//   * BC1/BC3 encodes run in one persistent kernel per interpolator present
-//     (k_color_batch); its waves take work units -- runs of consecutive blocks
-//     of one job -- from a queue in device memory, so textures of any size share
-//     the whole GPU and no block waits for a launch of its own;
+//   * BC1/BC3 encodes with endpoint search run in one persistent kernel per
+//     interpolator present (k_color_batch); its waves take work units -- runs of
+//     consecutive blocks of one job -- from a queue in device memory, so textures
+//     of any size share the whole GPU and no block waits for a launch of its own;
+//   * BC1/BC3 encodes without endpoint search run one block per lane
+//     (k_lane_batch), a workgroup per 256 blocks;
 //   * BC4/BC5 encodes and every decode run in one kernel with a workgroup per
 //     256 blocks (k_light_batch).
 // The kernels are independent branches of the graph.  Jobs are ordered by
@@ -25,6 +27,7 @@
 #include "kernels.h"
 #include "qtx_internal.h"
 #include "batch_internal.h"
+#include "bc1_lane.h"
 #include "runtime.h"
 
 using namespace qtx;
@@ -86,7 +89,7 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
                 return jfail(i, QTX_EINVAL, "BC3 colour blocks are always FourColor");
             b.kind = j.format == QTX_FMT_BC1 ? BJ_ENC_BC1 : BJ_ENC_BC3;
             b.interp = interp;
-            p.group = interp;
+            p.group = (dev::lane::lane_settings((int)b.s.search_rounds, b.s.ep_passes) ? kGroupLane : kGroupColour) + interp;
             p.block_ops = bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
         } else {
             if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
@@ -94,13 +97,13 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
             b.kind = j.format == QTX_FMT_BC4 ? BJ_ENC_BC4 : BJ_ENC_BC5;
             b.p0 = j.chan0;
             b.p1 = j.format == QTX_FMT_BC5 ? j.chan1 : 0;
-            p.group = 4;
+            p.group = kGroupLight;
             p.block_ops = j.format == QTX_FMT_BC4 ? 300.0 : 600.0;
         }
     } else if (j.op == QTX_OP_DECODE) {
         p.src_bytes = block_bytes(j.width, j.height, bpb);
         p.dst_bytes = raw_bytes(j.width, j.height);
-        p.group = 4;
+        p.group = kGroupLight;
         p.block_ops = 200.0;
         switch (j.format) {
             case QTX_FMT_BC1:
@@ -134,19 +137,20 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
     std::iota(idx.begin(), idx.end(), 0);
     auto total = [&](int i) { return jobs[i].block_ops * jobs[i].job.nblocks; };
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return total(x) > total(y); });
-    for (int grp = 0; grp <= 4; grp++) {
+    for (int grp = 0; grp < kGroups; grp++) {
         Plan::Group g;
-        g.interp = grp < 4 ? grp : -1;
+        g.kind = grp < kGroupLane ? kGroupColour : (grp < kGroupLight ? kGroupLane : kGroupLight);
+        g.interp = grp - g.kind;
         g.first = out.host.size();
         uint64_t unit = 0;
         for (int i : idx) {
             if (jobs[i].group != grp) continue;
             BatchJob j = jobs[i].job;
-            if (g.interp >= 0) {
+            if (g.kind == kGroupColour) {
                 const double rounds = std::round((double)t.unit_kops * 1000.0 / jobs[i].block_ops);
                 j.unit_blocks = 4u * (uint32_t)std::min<double>(std::max(rounds, 1.0), (double)kMaxRounds);
             } else {
-                j.unit_blocks = kLightUnitBlocks;
+                j.unit_blocks = g.kind == kGroupLane ? kLaneUnitBlocks : kLightUnitBlocks;
             }
             j.unit_begin = (uint32_t)unit;
             unit += (j.nblocks + j.unit_blocks - 1) / j.unit_blocks;
@@ -156,7 +160,7 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
         }
         if (g.order.empty()) continue;
         g.nunits = (uint32_t)unit;
-        if (g.interp >= 0) {
+        if (g.kind == kGroupColour) {
             const long long waves_needed = ((long long)g.nunits + 3) / 4;  // four waves per workgroup
             g.grid = (unsigned)std::max(1LL, std::min<long long>((long long)cu_count * t.workgroups_per_cu,
                                                                  waves_needed));
@@ -171,7 +175,9 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
 hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t *d_queue,
                          const DeviceTables *d_tables, hipStream_t s) {
     const BatchJob *dj = d_jobs + g.first;
-    if (g.interp >= 0) return launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_queue, g.grid, d_tables, s);
+    if (g.kind == kGroupColour)
+        return launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_queue, g.grid, d_tables, s);
+    if (g.kind == kGroupLane) return launch_lane_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_tables, s);
     return launch_light_batch(dj, (int)g.order.size(), g.nunits, s);
 }
 

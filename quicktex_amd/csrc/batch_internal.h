@@ -15,18 +15,24 @@ namespace bt {
 constexpr int kMaxColourWgPerCu = 4;   // __launch_bounds__(256, 4) of the colour kernel
 constexpr uint32_t kMaxRounds = 1024;  // a colour unit holds at most 4 * kMaxRounds blocks
 
+// Kernel groups of a batch: BC1/BC3 jobs with endpoint search (the 16-lane
+// persistent encoder) by interpolator, BC1/BC3 jobs without it (the lane-per-block
+// encoder) by interpolator, and the light kernel.
+constexpr int kGroupColour = 0, kGroupLane = 4, kGroupLight = 8, kGroups = 9;
+
 // A validated job: its kernel descriptor, estimated cost and byte sizes.
 struct Prepared {
     BatchJob job;
     double block_ops;  // estimated ALU ops per block
-    int group;         // interpolator 0..3 for colour jobs, 4 for light jobs
+    int group;         // kernel group: kGroupColour + interp, kGroupLane + interp or kGroupLight
     size_t src_bytes, dst_bytes;
 };
 
 // Kernel-ready layout: descriptors in group order, one queue per group.
 struct Plan {
     struct Group {
-        int interp = -1;         // 0..3 colour, -1 light
+        int kind = kGroupLight;  // kGroupColour, kGroupLane or kGroupLight
+        int interp = 0;
         std::vector<int> order;  // indices into the prepared list, descending total cost
         size_t first = 0;        // offset of the group's descriptors in host[]
         uint32_t nunits = 0;

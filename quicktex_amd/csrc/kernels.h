@@ -22,6 +22,11 @@ hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint
 constexpr uint32_t kLightUnitBlocks = 256;
 hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, uint32_t *d_queue,
                               unsigned grid, const DeviceTables *d_tables, hipStream_t stream);
+// BC1/BC3 jobs without endpoint search of one interpolator: one block per lane,
+// 256 blocks per unit, one workgroup per unit.
+constexpr uint32_t kLaneUnitBlocks = 256;
+hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits,
+                             const DeviceTables *d_tables, hipStream_t stream);
 hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, hipStream_t stream);
 
 }  // namespace qtx

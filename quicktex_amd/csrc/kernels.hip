@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "bc1_device.h"
+#include "bc1_lane.h"
 #include "kernels.h"
 
 namespace qtx {
@@ -119,6 +120,53 @@ __global__ __launch_bounds__(256, 4) void k_bc1_encode(const uint8_t *__restrict
         if (gid >= nblocks) break;
         encode_color_block<INTERP>(rgba, w, h, bw, gid, out, T, BC3);
     }
+}
+
+// The 16 texels of block gid (index x + 4y) for one lane: four 16-byte row loads
+// inside the texture, the wrap rule at its edges.
+__device__ __forceinline__ void load_block16(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+                                             uint32_t (&px)[16]) {
+    const int bx = gid % bw, by = gid / bw;
+    if ((w & 3) == 0 && bx * 4 + 3 < w && by * 4 + 3 < h) {
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const uint4 row = *reinterpret_cast<const uint4 *>(rgba + 4 * ((size_t)(by * 4 + y) * w + bx * 4));
+            px[4 * y + 0] = row.x; px[4 * y + 1] = row.y; px[4 * y + 2] = row.z; px[4 * y + 3] = row.w;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; t++) px[t] = load_texel(rgba, w, h, bx, by, t);
+    }
+}
+
+// One BC1 (or BC3) block per lane: encode and store.
+template <int INTERP>
+__device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+                                                  uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                  const BC1Settings &s, bool bc3) {
+    uint32_t px[16];
+    load_block16(rgba, w, h, bw, gid, px);
+    const uint2 colour = lane::encode_bc1<INTERP>(px, s, T);
+    if (bc3) {
+        uint32_t a[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) a[i] = px[i] >> 24;
+        const uint2 alpha = bc4_encode_values(a);
+        reinterpret_cast<uint4 *>(out)[gid] = make_uint4(alpha.x, alpha.y, colour.x, colour.y);
+    } else {
+        reinterpret_cast<uint2 *>(out)[gid] = colour;
+    }
+}
+
+// BC1/BC3 encode, one block per lane, for settings without endpoint search.
+template <int INTERP, bool BC3>
+__global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
+                                                  uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                  BC1Settings s) {
+    lane::load_tables<INTERP>(T);
+    __syncthreads();
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gid < nblocks) encode_lane_block<INTERP>(rgba, w, h, bw, gid, out, T, s, BC3);
 }
 
 // BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
@@ -357,6 +405,30 @@ __global__ __launch_bounds__(256, 4) void k_color_batch(const BatchJob *__restri
     retire(queue, gridDim.x * kWaves);
 }
 
+// Job of the workgroup's unit: binary search on unit_begin.
+__device__ __forceinline__ const BatchJob &unit_job(const BatchJob *__restrict__ jobs, int njobs, uint32_t u) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].unit_begin <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+// BC1/BC3 jobs without endpoint search of one interpolator: one block per lane,
+// one unit of 256 blocks per workgroup.
+template <int INTERP>
+__global__ __launch_bounds__(256, 4) void k_lane_batch(const BatchJob *__restrict__ jobs, int njobs,
+                                                    const DeviceTables *__restrict__ T) {
+    lane::load_tables<INTERP>(T);
+    __syncthreads();
+    const uint32_t u = blockIdx.x;
+    const BatchJob &J = unit_job(jobs, njobs, u);
+    const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
+    if (gid < J.nblocks) encode_lane_block<INTERP>(J.src, J.w, J.h, J.bw, (int)gid, J.dst, T, J.s, J.kind == BJ_ENC_BC3);
+}
+
 // BC4/BC5 encode and every decoder of a batch: one lane per block, one unit of
 // 256 blocks per workgroup.
 template <int KIND>
@@ -375,13 +447,7 @@ __device__ __forceinline__ void decode_interp(const BatchJob &J, int gid) {
 
 __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict__ jobs, int njobs) {
     const uint32_t u = blockIdx.x;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].unit_begin <= u) lo = mid;
-        else hi = mid - 1;
-    }
-    const BatchJob &J = jobs[lo];
+    const BatchJob &J = unit_job(jobs, njobs, u);
     const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
     if (gid >= J.nblocks) return;
     switch (J.kind) {
@@ -402,6 +468,7 @@ __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict_
 namespace {
 constexpr int kThreads = 256;
 constexpr size_t kBc1Lds = sizeof(dev::Lds);
+constexpr size_t kLaneLds = sizeof(dev::lane::LaneLds);
 // persistent workgroups for the BC1/BC3 encoder: 256 CUs x 4 resident workgroups
 constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
@@ -411,6 +478,27 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
                              const BC1Settings &s, int interp, bool bc3, hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
+    if (dev::lane::lane_settings((int)s.search_rounds, s.ep_passes)) {
+        const dim3 lgrid(grid_for(nblocks, kThreads)), lblock(kThreads);
+#define QTX_LANE_CASE(I)                                                                                            \
+    case I:                                                                                                         \
+        if (bc3)                                                                                                    \
+            hipLaunchKernelGGL((dev::k_bc1_lane<I, true>), lgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, nblocks, \
+                               d_out, d_tables, s);                                                                 \
+        else                                                                                                        \
+            hipLaunchKernelGGL((dev::k_bc1_lane<I, false>), lgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, nblocks,\
+                               d_out, d_tables, s);                                                                 \
+        break;
+        switch (interp) {
+            QTX_LANE_CASE(QTX_INTERP_IDEAL)
+            QTX_LANE_CASE(QTX_INTERP_IDEAL_ROUND)
+            QTX_LANE_CASE(QTX_INTERP_NVIDIA)
+            QTX_LANE_CASE(QTX_INTERP_AMD)
+            default: return hipErrorInvalidValue;
+        }
+#undef QTX_LANE_CASE
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)std::min<long long>(((long long)nblocks + 15) / 16, kBc1MaxGrid)), block(kThreads);
 #define QTX_BC1_CASE(I)                                                                                             \
     case I:                                                                                                         \
@@ -487,6 +575,28 @@ hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uin
         default: return hipErrorInvalidValue;
     }
 #undef QTX_CB
+    return hipGetLastError();
+}
+
+hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits,
+                             const DeviceTables *d_tables, hipStream_t stream) {
+    const dim3 grid(nunits), block(kThreads);
+    switch (interp) {
+        case QTX_INTERP_IDEAL:
+            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_IDEAL>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
+            break;
+        case QTX_INTERP_IDEAL_ROUND:
+            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_IDEAL_ROUND>), grid, block, kLaneLds, stream, d_jobs, njobs,
+                               d_tables);
+            break;
+        case QTX_INTERP_NVIDIA:
+            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_NVIDIA>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
+            break;
+        case QTX_INTERP_AMD:
+            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_AMD>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

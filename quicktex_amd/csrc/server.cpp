@@ -41,7 +41,7 @@ using namespace qtx::rt;
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr int kQueueWords = 2 * 5;  // two words per kernel group, at most five groups
+constexpr int kQueueWords = 2 * bt::kGroups;  // two words per kernel group
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
 
