@@ -27,7 +27,6 @@
 #include "kernels.h"
 #include "qtx_internal.h"
 #include "batch_internal.h"
-#include "bc1_lane.h"
 #include "runtime.h"
 
 using namespace qtx;
@@ -89,7 +88,7 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
                 return jfail(i, QTX_EINVAL, "BC3 colour blocks are always FourColor");
             b.kind = j.format == QTX_FMT_BC1 ? BJ_ENC_BC1 : BJ_ENC_BC3;
             b.interp = interp;
-            p.group = (dev::lane::lane_settings((int)b.s.search_rounds, b.s.ep_passes) ? kGroupLane : kGroupColour) + interp;
+            p.group = (use_lane_encoder(b.s) ? kGroupLane : kGroupColour) + interp;
             p.block_ops = bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
         } else {
             if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))

@@ -552,14 +552,44 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     return best;
 }
 
-// True when the lane encoder handles these settings: no endpoint search and one
-// endpoint pass (levels 0-12).
-__host__ __device__ constexpr bool lane_settings(int search_rounds, int ep_passes) {
-    return search_rounds == 0 && ep_passes == 1;
+// EndpointSearch (BC1Encoder.cpp:930-996), the reference's sequential loop: round i
+// moves the low (i & 16) or high endpoint by voxel i & 15, skips the reverse of the
+// last improving move, and stops 33 rounds after the last improvement.
+template <int INTERP>
+__device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const Blk &bk, uint32_t rounds,
+                                               int four_error_mode, Res r) {
+    const bool three = r.mode == QTX_COLOR_THREE || r.mode == QTX_COLOR_THREE_BLACK;
+    const int M = three ? r.mode : QTX_COLOR_FOUR;
+    const int emode = three ? QTX_ERROR_FULL : four_error_mode;
+    uint32_t prev = 0;
+    int forbidden = -1;
+    for (uint32_t i = 0; i < rounds; i++) {
+        if ((int)(i & 31u) == forbidden) continue;
+        const uint32_t v = i & 15u;
+        const uint32_t e = (i & 16u) ? (r.ep & 0xFFFFu) : (r.ep >> 16);
+        const uint32_t ne = ep_make((uint32_t)clampi((int)ep_r(e) + vox_d(kVoxRp, v), 0, 31),
+                                    (uint32_t)clampi((int)ep_g(e) + vox_d(kVoxGp, v), 0, 63),
+                                    (uint32_t)clampi((int)ep_b(e) + vox_d(kVoxBp, v), 0, 31));
+        const uint32_t ep = (i & 16u) ? ((r.ep & 0xFFFF0000u) | ne) : ((r.ep & 0xFFFFu) | (ne << 16));
+        uint32_t sel;
+        const uint32_t err = find_selectors<INTERP>(px, bk, M, emode, ep, sel);
+        if (err < r.err) {
+            r = Res{ep, sel, err, M};
+            forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
+            prev = i;
+        }
+        if (i - prev > 32u) break;
+    }
+    return r;
 }
 
+// True when the lane encoder is the faster of the two for these settings: all
+// but the exhaustive cluster fit (level 19), whose 969 + 153 orderings per pass
+// the 16-lane encoder deals to 16 lanes.
+__host__ __device__ constexpr bool lane_settings(int exhaustive) { return exhaustive == 0; }
+
 // BC1Encoder::EncodeBlock (BC1Encoder.cpp:261-342) for one block (texels RGBA8,
-// index x + 4y); lane_settings(s.search_rounds, s.ep_passes) holds.
+// index x + 4y), any settings.
 template <int INTERP>
 __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC1Settings &s,
                                             const DeviceTables *__restrict__ T) {
@@ -621,6 +651,10 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
         t.err = find_selectors<INTERP>(px, bk, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, ep, t.sel);
         t = refine_block_ls<INTERP>(px, bk, mnb, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, s.ls_passes, t);
         if (t.err < result.err) result = t;
+    }
+    if (result.err > 0 && s.search_rounds > 0) {
+        touch(px);
+        result = endpoint_search<INTERP>(px, bk, s.search_rounds, s.error_mode, result);
     }
     return write_block(result);
 }

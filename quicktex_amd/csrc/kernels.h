@@ -4,7 +4,24 @@
 
 #include "qtx_internal.h"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace qtx {
+
+// Which BC1/BC3 encoder serves a setting: QTX_BC1_PATH=lane or =group forces one
+// (both give identical bytes); unset or anything else picks per setting.
+enum Bc1Path { BC1_AUTO = 0, BC1_LANE = 1, BC1_GROUP = 2 };
+inline Bc1Path bc1_path_override() {
+    static const Bc1Path p = [] {
+        const char *e = std::getenv("QTX_BC1_PATH");
+        if (e && !std::strcmp(e, "lane")) return BC1_LANE;
+        if (e && !std::strcmp(e, "group")) return BC1_GROUP;
+        return BC1_AUTO;
+    }();
+    return p;
+}
+bool use_lane_encoder(const BC1Settings &s);
 
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
                              const BC1Settings &s, int interp, bool bc3, hipStream_t stream);

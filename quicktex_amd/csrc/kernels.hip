@@ -474,11 +474,17 @@ constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
 }  // namespace
 
+bool use_lane_encoder(const BC1Settings &s) {
+    const Bc1Path p = bc1_path_override();
+    if (p != BC1_AUTO) return p == BC1_LANE;
+    return dev::lane::lane_settings(s.exhaustive);
+}
+
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
                              const BC1Settings &s, int interp, bool bc3, hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
-    if (dev::lane::lane_settings((int)s.search_rounds, s.ep_passes)) {
+    if (use_lane_encoder(s)) {
         const dim3 lgrid(grid_for(nblocks, kThreads)), lblock(kThreads);
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
