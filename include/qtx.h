@@ -90,6 +90,15 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx);
 /* Blocks until all work the context enqueued on its own stream is done. */
 qtx_status qtx_ctx_synchronize(qtx_ctx *ctx);
 
+/* Which of the two BC1/BC3 encoders serves this context's encodes: QTX_BC1_AUTO
+ * (default) picks by setting and amount of work, QTX_BC1_LANE forces the
+ * one-block-per-lane encoder (any non-exhaustive setting), QTX_BC1_GROUP the
+ * 16-lanes-per-block encoder.  Both produce identical bytes; this is a speed
+ * knob (and lets tests cover both).  Applies to later plain launches and batches
+ * and servers created afterwards.  No counterpart in the reference. */
+enum { QTX_BC1_AUTO = 0, QTX_BC1_LANE = 1, QTX_BC1_GROUP = 2 };
+qtx_status qtx_ctx_set_bc1_encoder(qtx_ctx *ctx, int32_t which);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident entry points (inputs and outputs already in HBM).       */
 

@@ -36,9 +36,13 @@ namespace {
 
 // ----------------------------------------------------------------------------
 // Algorithmic work accounting.  Each primitive of the encoder counts its
-// invocations; qtxo_op_weights() gives the static number of scalar integer /
-// FP32 operations one invocation of a straightforward implementation needs
-// (16 texels, the reference's formulation).  ops = sum(count * weight) is the
+// invocations; qtxo_op_weights() gives the static number of 32-bit lane
+// operations one invocation needs on a GPU lane (16 texels, the reference's
+// formulation), counting a dot product of two RGB byte triples as one operation
+// (the hardware has it as one instruction, v_dot4_u32_u8) and every other add,
+// multiply, compare, select, conversion, shift or table read as one.  A selector
+// search is priced by what its error needs (the selectors of a trial that is not
+// kept are not work the algorithm requires).  ops = sum(count * weight) is the
 // "algorithmic ops" of a workload.  Counts are per thread: count single-threaded.
 enum OpKind {
     OP_METRICS, OP_SOLID, OP_EP_GREY, OP_EP_LS, OP_EP_BB, OP_EP_BBINT, OP_EP_PCA, OP_PCA_ITER,
@@ -46,23 +50,23 @@ enum OpKind {
     OP_LS_PX, OP_LS_SUMS, OP_SC_PX, OP_CF_SETUP, OP_SEARCH_ROUND, OP_WRITE, OP_BC4, OP_KINDS
 };
 const uint32_t kOpWeight[OP_KINDS] = {
-    280,  // metrics: per texel min/max/sum/grey/black tests
+    290,  // metrics: 16 x (3 extracts, 3 min, 3 max, 3 adds, grey and black tests) + 3 averages
     30,   // solid block: 3 table lookups, packing, order fix-up
     15,   // greyscale endpoints
-    146,  // least-squares endpoints: 16 x (3 mul + 3 add) + solve
-    136,  // bounding box (float): insets, 16 x covariance sign terms
-    121,  // bounding box (int)
-    421,  // PCA: covariance 16 x 15, scaling, 16 projections x 10
+    150,  // least-squares endpoints: 16 x (3 extracts, 3 mul-adds) + solve + 2 roundings
+    130,  // bounding box (float): insets, 16 x covariance sign terms, 2 roundings
+    115,  // bounding box (int)
+    360,  // PCA: covariance 16 x 9, scaling, 16 projections x 11
     32,   // one PCA power iteration: 4 rows x (4 mul + 4 add)
-    702,  // 4-candidate full selector search: palette + 16 x (4 x 10 + 2)
-    542,  // 3-candidate full selector search
-    568,  // Check2 selector search: 16 x 33 + setup
-    365,  // Faster selector search: 16 x 20 + setup
-    237,  // selector search without error
-    226,  // least squares from texels: 16 x 10 + solve + 2 roundings
-    70,   // least squares from prefix sums + 2 roundings
-    216,  // single-colour fallback error: palette + 16 x 11
-    310,  // cluster-fit set-up: histogram, 16 projections, sort, prefix sums
+    232,  // 4-candidate full selector search: palette 40 + 16 x (4 dots, 4 mul-subs, 3 max, 1 add)
+    184,  // 3-candidate full selector search: palette 40 + 16 x 9
+    338,  // Check2 selector search: set-up 50 + 16 x 18
+    306,  // Faster selector search: set-up 50 + 16 x 16
+    210,  // selector search without error: set-up 50 + 16 x 10
+    200,  // least squares from texels: 16 x 8 + solve + 2 roundings
+    80,   // least squares from prefix sums: 3 reads, sums, 2 x 3 mul-adds, 2 roundings
+    85,   // single-colour fallback error: palette + 16 dots + sums + 3 table reads
+    430,  // cluster-fit set-up: histogram, 16 projections, 63-comparator sort, prefix sums
     8,    // one endpoint-search move
     42,   // block packing
     180,  // one BC4 channel block

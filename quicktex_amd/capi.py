@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, 'libqtx.so')
 INTERP_IDEAL, INTERP_IDEAL_ROUND, INTERP_NVIDIA, INTERP_AMD = 0, 1, 2, 3
 COLOR_FOUR, COLOR_THREE, COLOR_THREE_BLACK = 4, 3, 0x13
 FMT_BC1, FMT_BC3, FMT_BC4, FMT_BC5 = 1, 3, 4, 5
+BC1_AUTO, BC1_LANE, BC1_GROUP = 0, 1, 2
 
 
 class BC1Params(ctypes.Structure):
@@ -114,6 +115,7 @@ class Qtx:
         L.qtx_ctx_synchronize.argtypes = [vp]
         L.qtx_bc1_params_init.argtypes = [P, u32, i32, i32]
         L.qtx_bc1_params_clamp.argtypes = [P]
+        L.qtx_ctx_set_bc1_encoder.argtypes = [vp, i32]
         for n in ('qtx_bc1_encode', 'qtx_bc3_encode'):
             getattr(L, n).argtypes = [vp, P, _U8P, i32, i32, _U8P, sz]
         L.qtx_bc4_encode.argtypes = [vp, u32, _U8P, i32, i32, _U8P, sz]
@@ -166,6 +168,10 @@ class Qtx:
         if self._ctx is not None:
             self.lib.qtx_ctx_destroy(self._ctx)
             self._ctx = None
+
+    def set_bc1_encoder(self, which):
+        """BC1_AUTO, BC1_LANE or BC1_GROUP for this context's later BC1/BC3 encodes."""
+        self.check(self.lib.qtx_ctx_set_bc1_encoder(self.ctx, which))
 
     def synchronize(self):
         self.check(self.lib.qtx_ctx_synchronize(self.ctx))

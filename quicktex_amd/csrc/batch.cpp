@@ -35,20 +35,6 @@ using namespace qtx::rt;
 namespace qtx {
 namespace bt {
 
-// Estimated ALU ops per BC1 block for the encoder settings: a fit of the
-// oracle's op accounting on the block mixture (levels 0-19): about 2k ops of
-// metrics, endpoints and selectors, 500 per cluster-fit ordering and pass,
-// and 780 per endpoint-search round, of which about 37 run before the search
-// stops for lack of improvement.
-double bc1_block_ops(const BC1Settings &s) {
-    const double o4 = s.exhaustive ? kOrders4 : s.orderings4;
-    const double o3 = s.exhaustive ? kOrders3 : s.orderings3;
-    const bool three = (s.color_mode & QTX_COLOR_THREE) != 0;
-    double ops = 2000.0 + s.cf_passes * (o4 * 500.0 + (three ? o3 * 500.0 + 1000.0 : 0.0));
-    ops += 780.0 * std::min<double>(s.search_rounds, 37.0);
-    return ops;
-}
-
 // Failure of job i, or of a single request when i is kNoIndex: the message is
 // prefixed with "job <i>: " for batches.
 static qtx_status jfail(size_t i, qtx_status st, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -88,8 +74,8 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
                 return jfail(i, QTX_EINVAL, "BC3 colour blocks are always FourColor");
             b.kind = j.format == QTX_FMT_BC1 ? BJ_ENC_BC1 : BJ_ENC_BC3;
             b.interp = interp;
-            p.group = (use_lane_encoder(b.s) ? kGroupLane : kGroupColour) + interp;
-            p.block_ops = bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
+            p.group = kGroupColour + interp;  // plan() moves it to the lane encoder when that is faster
+            p.block_ops = qtx::bc1_block_ops(b.s) + (j.format == QTX_FMT_BC3 ? 300.0 : 0.0);
         } else {
             if (j.chan0 < 0 || j.chan0 > 3 || (j.format == QTX_FMT_BC5 && (j.chan1 < 0 || j.chan1 > 3)))
                 return jfail(i, QTX_EINVAL, "Channel out of range");
@@ -129,9 +115,21 @@ qtx_status prepare(const qtx_job &j, size_t i, Prepared &p) {
 
 // Unit layout of a list of prepared jobs: groups (colour by interpolator, then
 // light), each in descending total cost, unit sizes for the tuning.
-qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, Plan &out) {
+qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, int bc1_path, Plan &out) {
     out.host.clear();
     out.groups.clear();
+    // BC1/BC3 jobs of an interpolator go to the lane encoder when there is enough
+    // such work in the launch (use_lane_encoder with the group's total blocks)
+    uint64_t colour_blocks[4] = {0, 0, 0, 0};
+    for (const Prepared &p : jobs)
+        if (p.group >= kGroupColour && p.group < kGroupLane) colour_blocks[p.group - kGroupColour] += p.job.nblocks;
+    std::vector<int> group(jobs.size());
+    for (size_t i = 0; i < jobs.size(); i++) {
+        const int g = jobs[i].group;
+        group[i] = g;
+        if (g >= kGroupColour && g < kGroupLane && use_lane_encoder(jobs[i].job.s, colour_blocks[g - kGroupColour], bc1_path))
+            group[i] = kGroupLane + (g - kGroupColour);
+    }
     std::vector<int> idx(jobs.size());
     std::iota(idx.begin(), idx.end(), 0);
     auto total = [&](int i) { return jobs[i].block_ops * jobs[i].job.nblocks; };
@@ -143,7 +141,7 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
         g.first = out.host.size();
         uint64_t unit = 0;
         for (int i : idx) {
-            if (jobs[i].group != grp) continue;
+            if (group[i] != grp) continue;
             BatchJob j = jobs[i].job;
             if (g.kind == kGroupColour) {
                 const double rounds = std::round((double)t.unit_kops * 1000.0 / jobs[i].block_ops);
@@ -192,6 +190,7 @@ struct qtx_batch {
     BatchJob *d_jobs = nullptr;
     uint32_t *d_queues = nullptr;  // two words per group
     qtx_batch_tuning tuning{kMaxColourWgPerCu, 256};
+    int bc1_path = QTX_BC1_AUTO;
     hipStream_t cap = nullptr;
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> joins;
@@ -211,7 +210,7 @@ void drop_graph(qtx_batch *b) {
 
 // Unit layout for the tuning, upload of the job array, capture of the graph.
 qtx_status rebuild(qtx_batch *b) {
-    qtx_status st = plan(b->jobs, b->tuning, b->ctx->cu_count, b->plan);
+    qtx_status st = plan(b->jobs, b->tuning, b->ctx->cu_count, b->bc1_path, b->plan);
     if (st != QTX_OK) return st;
     const std::vector<BatchJob> &host = b->plan.host;
     if (b->done) QTX_HIP(hipEventSynchronize(b->done));
@@ -278,6 +277,7 @@ qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx
     if (njobs > (1u << 24)) return fail(QTX_ERANGE, "batch of %zu jobs exceeds 2^24", njobs);
     auto *b = new qtx_batch;
     b->ctx = ctx;
+    b->bc1_path = ctx->bc1_path;
     b->jobs.resize(njobs);
     for (size_t i = 0; i < njobs; i++) {
         if ((st = prepare(jobs[i], i, b->jobs[i])) != QTX_OK) {
@@ -285,7 +285,7 @@ qtx_status qtx_batch_create(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, qtx
             return st;
         }
     }
-    if ((st = plan(b->jobs, b->tuning, ctx->cu_count, b->plan)) != QTX_OK) {
+    if ((st = plan(b->jobs, b->tuning, ctx->cu_count, b->bc1_path, b->plan)) != QTX_OK) {
         delete b;
         return st;
     }

@@ -42,12 +42,11 @@ struct Plan {
     std::vector<Group> groups;  // colour groups by interpolator, then the light group
 };
 
-// Estimated ALU ops per BC1 block for the encoder settings.
-double bc1_block_ops(const BC1Settings &s);
 constexpr size_t kNoIndex = ~(size_t)0;
 // Validates job i (messages start "job <i>: "; none for i == kNoIndex) and fills p.
 qtx_status prepare(const qtx_job &j, size_t i, Prepared &p);
-qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, Plan &out);
+// bc1_path: QTX_BC1_AUTO / _LANE / _GROUP (the context's choice)
+qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, int cu_count, int bc1_path, Plan &out);
 // Launches one group's kernel; d_jobs holds plan.host, d_queue the group's two
 // queue words (zero before the first launch; the kernels leave them zero).
 hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t *d_queue,

@@ -370,6 +370,59 @@ __device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], con
     return bk.pp - (uint32_t)tot;
 }
 
+// The error find_selectors returns, without the selectors: what cluster-fit and
+// endpoint-search trials need (the winner's selectors are computed once).
+template <int INTERP>
+__device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
+                                                uint32_t ep) {
+    uint32_t c[4];
+    palette_pair<INTERP>(ep, (M & 0xF) == 3, c);
+    const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
+    const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
+    int tot = 0;
+    if (error_mode == QTX_ERROR_FULL) {
+        if (M == QTX_COLOR_THREE) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t p = px[i];
+                const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
+                const int u2 = 2 * (int)dot4(p, c[2]) - cc2;
+                tot += max(max(u0, u1), u2);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t p = px[i];
+                const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
+                const int u2 = 2 * (int)dot4(p, c[2]) - cc2, u3 = 2 * (int)dot4(p, c[3]) - cc3;
+                tot += max(max(u0, u1), max(u2, u3));
+            }
+        }
+        return bk.pp - (uint32_t)tot;
+    }
+    if (error_mode == QTX_ERROR_CHECK2) {
+        const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
+        const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
+        const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
+        const float f = 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+        const int c0a = (int)dot4(c[0], c[3]) - cc0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t p = px[i];
+            const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
+            const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
+            const int k = clampi(cvt_trunc((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);
+            const int u0 = 2 * d0 - cc0, u1 = 2 * d1 - cc1, u2 = 2 * d2 - cc2, u3 = 2 * d3 - cc3;
+            const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);
+            const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);
+            tot += max(ua, ub);
+        }
+        return bk.pp - (uint32_t)tot;
+    }
+    uint32_t sel;
+    return find_selectors<INTERP>(px, bk, M, error_mode, ep, sel);
+}
+
 // FindEndpointsSingleColor(result, pixels, color, is_3color) (BC1Encoder.cpp:458-471):
 // match-table endpoints, error against colour 2 of the endpoints held on entry
 // (alpha included: that colour's alpha is 255), all selectors 1.
@@ -519,13 +572,18 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     touch(px);
     const uint32_t q_total = exhaustive ? (four ? (uint32_t)kOrders4 : (uint32_t)kOrders3) : orderings;
     const uint16_t *best_row = four ? &T->best4[start * kBestOrders4] : &T->best3[start * kBestOrders3];
-    Res best = orig;
+    // trials keep (error, endpoints, single-colour flag); the winner's selectors
+    // are computed once at the end
+    uint32_t b_err = orig.err, b_ep = orig.ep;
+    bool b_single = false, improved = false;
     for (uint32_t q = 0; q < q_total; q++) {
         const uint32_t hash = exhaustive ? q : best_row[q];
         const uint32_t lv = four ? T->levels4[hash] : T->levels3[hash];
-        Res t;
+        uint32_t t_err, t_ep;
         if (lv >> 24) {
-            t = single_color_px<INTERP>(px, bk, orig.ep, m.avg, !four);
+            const Res t = single_color_px<INTERP>(px, bk, orig.ep, m.avg, !four);
+            t_err = t.err;
+            t_ep = t.ep;
         } else {
             // RefineEndpointsLS(result, sums, matrix, hash) (:827-850):
             // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10, exact integers
@@ -542,13 +600,21 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
                                         (fm.x * q00_2) + (fm.y * q10_2));
             const uint32_t hi = prnd565((fm.z * q00_0) + (fm.w * q10_0), (fm.z * q00_1) + (fm.w * q10_1),
                                         (fm.z * q00_2) + (fm.w * q10_2));
-            t.ep = lo | (hi << 16);
-            t.err = find_selectors<INTERP>(px, bk, M, error_mode, t.ep, t.sel);
-            t.mode = M;
+            t_ep = lo | (hi << 16);
+            t_err = trial_error<INTERP>(px, bk, M, error_mode, t_ep);
         }
-        if (t.err < best.err) best = t;
-        if (t.err == 0) break;
+        if (t_err < b_err) {
+            b_err = t_err;
+            b_ep = t_ep;
+            b_single = (lv >> 24) != 0;
+            improved = true;
+        }
+        if (t_err == 0) break;
     }
+    if (!improved) return orig;
+    if (b_single) return Res{b_ep, 0x55555555u, b_err, four ? QTX_COLOR_FOUR : QTX_COLOR_THREE};
+    Res best{b_ep, 0u, b_err, M};
+    (void)find_selectors<INTERP>(px, bk, M, error_mode, b_ep, best.sel);
     return best;
 }
 
@@ -571,10 +637,10 @@ __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const B
                                     (uint32_t)clampi((int)ep_g(e) + vox_d(kVoxGp, v), 0, 63),
                                     (uint32_t)clampi((int)ep_b(e) + vox_d(kVoxBp, v), 0, 31));
         const uint32_t ep = (i & 16u) ? ((r.ep & 0xFFFF0000u) | ne) : ((r.ep & 0xFFFFu) | (ne << 16));
-        uint32_t sel;
-        const uint32_t err = find_selectors<INTERP>(px, bk, M, emode, ep, sel);
+        const uint32_t err = trial_error<INTERP>(px, bk, M, emode, ep);
         if (err < r.err) {
-            r = Res{ep, sel, err, M};
+            r = Res{ep, 0u, err, M};
+            (void)find_selectors<INTERP>(px, bk, M, emode, ep, r.sel);
             forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
             prev = i;
         }

@@ -199,6 +199,14 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
     return QTX_OK;
 }
 
+qtx_status qtx_ctx_set_bc1_encoder(qtx_ctx *ctx, int32_t which) {
+    if (!ctx) return fail(QTX_EINVAL, "null context");
+    if (which != QTX_BC1_AUTO && which != QTX_BC1_LANE && which != QTX_BC1_GROUP)
+        return fail(QTX_EINVAL, "unknown BC1 encoder %d", which);
+    ctx->bc1_path = which;
+    return QTX_OK;
+}
+
 qtx_status qtx_ctx_synchronize(qtx_ctx *ctx) {
     qtx_status st = check_ctx(ctx);
     if (st != QTX_OK) return st;
@@ -218,7 +226,7 @@ qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     int interp = 0;
     if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
-    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false,
+    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false, ctx->bc1_path,
                                    (hipStream_t)stream));
     return QTX_OK;
 }
@@ -233,7 +241,7 @@ qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
     if (s.color_mode != QTX_COLOR_FOUR) return fail(QTX_EINVAL, "BC3 colour blocks are always FourColor");
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
-    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, true,
+    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, true, ctx->bc1_path,
                                    (hipStream_t)stream));
     return QTX_OK;
 }

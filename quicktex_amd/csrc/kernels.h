@@ -9,22 +9,31 @@
 
 namespace qtx {
 
-// Which BC1/BC3 encoder serves a setting: QTX_BC1_PATH=lane or =group forces one
-// (both give identical bytes); unset or anything else picks per setting.
-enum Bc1Path { BC1_AUTO = 0, BC1_LANE = 1, BC1_GROUP = 2 };
-inline Bc1Path bc1_path_override() {
-    static const Bc1Path p = [] {
+// Which BC1/BC3 encoder serves a setting (QTX_BC1_AUTO / _LANE / _GROUP): the
+// context's choice, else the environment (QTX_BC1_PATH=lane or =group), else
+// automatic.  Both encoders give identical bytes.
+inline int bc1_path_env() {
+    static const int p = [] {
         const char *e = std::getenv("QTX_BC1_PATH");
-        if (e && !std::strcmp(e, "lane")) return BC1_LANE;
-        if (e && !std::strcmp(e, "group")) return BC1_GROUP;
-        return BC1_AUTO;
+        if (e && !std::strcmp(e, "lane")) return (int)QTX_BC1_LANE;
+        if (e && !std::strcmp(e, "group")) return (int)QTX_BC1_GROUP;
+        return (int)QTX_BC1_AUTO;
     }();
     return p;
 }
-bool use_lane_encoder(const BC1Settings &s);
+// Estimated ALU ops per BC1 block for the encoder settings.
+double bc1_block_ops(const BC1Settings &s);
+// Whether the lane-per-block encoder serves settings s when `blocks` blocks of
+// such work are launched together (else the 16-lane encoder does): the lane
+// encoder needs enough blocks to fill the GPU, the 16-lane one spreads a block
+// over 16 lanes and wins on few blocks of costly settings.
+constexpr uint64_t kLaneFullBlocks = 262144;  // 256 CUs x 16 waves x 64 lanes
+constexpr uint64_t kLaneMinBlocks = 32768;
+constexpr double kLaneMaxOps = 120000.0;
+bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path);
 
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, hipStream_t stream);
+                             const BC1Settings &s, int interp, bool bc3, int path, hipStream_t stream);
 hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, int ch0, int ch1, bool bc5,
                              hipStream_t stream);
 // format: QTX_FMT_*; p0/p1: write_alpha for BC1, channels for BC4/BC5

@@ -474,17 +474,35 @@ constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
 }  // namespace
 
-bool use_lane_encoder(const BC1Settings &s) {
-    const Bc1Path p = bc1_path_override();
-    if (p != BC1_AUTO) return p == BC1_LANE;
-    return dev::lane::lane_settings(s.exhaustive);
+// A fit of the oracle's op accounting on the block mixture (levels 0-19): about 2k
+// ops of metrics, endpoints and selectors, 500 per cluster-fit ordering and pass,
+// and 780 per endpoint-search round, of which about 37 run before the search stops
+// for lack of improvement.
+double bc1_block_ops(const BC1Settings &s) {
+    const double o4 = s.exhaustive ? kOrders4 : s.orderings4;
+    const double o3 = s.exhaustive ? kOrders3 : s.orderings3;
+    const bool three = (s.color_mode & QTX_COLOR_THREE) != 0;
+    double ops = 2000.0 + s.cf_passes * (o4 * 500.0 + (three ? o3 * 500.0 + 1000.0 : 0.0));
+    ops += 780.0 * (s.search_rounds < 37u ? (double)s.search_rounds : 37.0);
+    return ops;
+}
+
+// Crossover measured on MI355X (ThreeColorBlack, block mixture): the lane encoder
+// wins from 65k blocks at levels <= 15 and from 262k blocks at level 18.
+bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
+    const int p = path != QTX_BC1_AUTO ? path : bc1_path_env();
+    if (p == QTX_BC1_GROUP) return false;
+    if (p == QTX_BC1_LANE) return dev::lane::lane_settings(s.exhaustive);
+    if (!dev::lane::lane_settings(s.exhaustive)) return false;
+    if (blocks >= kLaneFullBlocks) return true;
+    return blocks >= kLaneMinBlocks && bc1_block_ops(s) <= kLaneMaxOps;
 }
 
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, hipStream_t stream) {
+                             const BC1Settings &s, int interp, bool bc3, int path, hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
-    if (use_lane_encoder(s)) {
+    if (use_lane_encoder(s, (uint64_t)nblocks, path)) {
         const dim3 lgrid(grid_for(nblocks, kThreads)), lblock(kThreads);
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \

@@ -16,6 +16,7 @@ struct qtx_ctx {
     hipStream_t stream = nullptr;
     qtx::DeviceTables *d_tables = nullptr;
     int cu_count = 0;
+    int bc1_path = 0;  // QTX_BC1_AUTO / _LANE / _GROUP
     // staging for the host entry points
     uint8_t *d_in = nullptr;
     size_t in_cap = 0;

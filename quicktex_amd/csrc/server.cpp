@@ -87,6 +87,7 @@ struct Slot {
 struct qtx_server {
     qtx_ctx *ctx = nullptr;
     qtx_server_config cfg{};
+    int bc1_path = 0;
     uint8_t *d_arena = nullptr;
     uint8_t *h_stage = nullptr;
     srv::Arena arena, stage;
@@ -149,7 +150,7 @@ hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
     if (e != hipSuccess) return e;
     bt::Plan plan;
     const qtx_batch_tuning tuning{bt::kMaxColourWgPerCu, s->cfg.unit_kops};
-    if (bt::plan(prep, tuning, s->ctx->cu_count, plan) != QTX_OK) return hipErrorInvalidValue;
+    if (bt::plan(prep, tuning, s->ctx->cu_count, s->bc1_path, plan) != QTX_OK) return hipErrorInvalidValue;
     std::memcpy(sl.h_jobs, plan.host.data(), plan.host.size() * sizeof(BatchJob));
     e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st);
     for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++)
@@ -375,6 +376,7 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     auto *s = new qtx_server;
     s->ctx = ctx;
     s->cfg = c;
+    s->bc1_path = ctx->bc1_path;
     s->origin = std::chrono::steady_clock::now();
     hipError_t e = hipMalloc(&s->d_arena, c.device_bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_stage, c.staging_bytes, hipHostMallocDefault);

@@ -63,8 +63,10 @@ def _check(cases, outs, want, what):
             what, k, 'enc' if c.op == OP_ENCODE else 'dec', c.fmt, c.w, c.h, c.level, c.mode, c.interp)
 
 
-def test_batch_matches_alone_and_checker(qtx, stream, mixed):
+@pytest.mark.parametrize('encoder', [0, 1, 2])  # auto, lane, group
+def test_batch_matches_alone_and_checker(qtx, stream, mixed, encoder):
     cases, srcs, want = mixed
+    qtx.set_bc1_encoder(encoder)
     alone = _outs(cases)
     for c, s, o in zip(cases, srcs, alone):
         c.plain(qtx, s.ptr, o.ptr, stream.handle)
@@ -76,6 +78,7 @@ def test_batch_matches_alone_and_checker(qtx, stream, mixed):
     stream.synchronize()
     _check(cases, outs, want, 'batch')
     b.close()
+    qtx.set_bc1_encoder(0)
 
 
 def test_batch_relaunch_under_every_tuning(qtx, stream, mixed):

@@ -3,7 +3,8 @@
 The checker is the reference's own C++ (oracle/_ref) where it was built, else
 the oracle restatement.  Bit-exact for every byte: BC1 at every level (0-19),
 colour mode and interpolator on a mixture image and a stress image of
-two/three-colour, greyscale and near-black blocks; BC3 with varying alpha;
+two/three-colour, greyscale and near-black blocks, through each of the two BC1
+encoders (one block per lane; 16 lanes per block); BC3 with varying alpha;
 BC4/BC5; every decoder, on ragged sizes.
 
 Level 1 (EndpointMode.LeastSquares) with the Nvidia interpolator is compared
@@ -52,6 +53,17 @@ def images():
     return [synth.mixture_rgba(512, 32, seed=0x5EED0000 + 7, near_black=0.15), stress_rgba(256, 32, 11)]
 
 
+ENCODERS = {'lane': 1, 'group': 2}  # QTX_BC1_LANE, QTX_BC1_GROUP: both encoders, identical bytes
+
+
+@pytest.fixture(params=sorted(ENCODERS))
+def encoder(qtx, request):
+    """Runs the test once with each BC1/BC3 encoder forced on the context."""
+    qtx.set_bc1_encoder(ENCODERS[request.param])
+    yield request.param
+    qtx.set_bc1_encoder(0)
+
+
 @pytest.fixture(scope='module')
 def checker():
     from tests._cpu_libs import oracle, reference
@@ -61,7 +73,7 @@ def checker():
 @pytest.mark.parametrize('interp', INTERPS)
 @pytest.mark.parametrize('mode', MODES)
 @pytest.mark.parametrize('level', LEVELS)
-def test_bc1_encode_levels(qtx, oracle, checker, images, level, mode, interp):
+def test_bc1_encode_levels(qtx, oracle, checker, images, level, mode, interp, encoder):
     p = qtx.params(level, mode, interp)
     assert p.as_tuple() == oracle.params(level, mode, interp).as_tuple()
     cpu = oracle if (level == 1 and interp == 2) else checker
@@ -75,7 +87,7 @@ def test_bc1_encode_levels(qtx, oracle, checker, images, level, mode, interp):
 
 @pytest.mark.parametrize('interp', INTERPS)
 @pytest.mark.parametrize('level', [0, 2, 5, 10, 13, 18])
-def test_bc3_encode_alpha(qtx, checker, level, interp):
+def test_bc3_encode_alpha(qtx, checker, level, interp, encoder):
     img = synth.mixture_rgba(48, 40, seed=99)
     rng = np.random.default_rng(level * 4 + interp)
     img[..., 3] = rng.integers(0, 256, img.shape[:2], dtype=np.uint8)
@@ -88,7 +100,7 @@ def test_bc3_encode_alpha(qtx, checker, level, interp):
 
 
 @pytest.mark.parametrize('w,h', [(1, 1), (2, 3), (3, 5), (5, 5), (7, 9), (37, 23), (64, 4)])
-def test_bc1_encode_ragged(qtx, oracle, w, h):
+def test_bc1_encode_ragged(qtx, oracle, w, h, encoder):
     img = synth.mixture_rgba(w, h, seed=w * 100 + h, near_black=0.1)
     for level in (3, 18):
         p = qtx.params(level, 0x13, 0)

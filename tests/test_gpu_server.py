@@ -48,13 +48,16 @@ def submit_all(srv, cases, outs, threads=3):
     return tickets
 
 
-def test_server_mix_matches_alone(qtx, checker, oracle):
+@pytest.mark.parametrize('encoder', [0, 1])  # auto, lane
+def test_server_mix_matches_alone(qtx, checker, oracle, encoder):
     cases = make_cases(qtx, 160, 0x5E12)
     alone = [np.asarray(c.host_plain(qtx)).reshape(-1) for c in cases]
     # half the outputs in pinned memory (written directly), half pageable (staged)
     pinned = [hip.Pinned(c.out_bytes) if i % 2 else None for i, c in enumerate(cases)]
     outs = [p.array if p is not None else np.zeros(c.out_bytes, np.uint8) for p, c in zip(pinned, cases)]
+    qtx.set_bc1_encoder(encoder)
     srv = Server(qtx, device_bytes=64 << 20, staging_bytes=32 << 20, slots=3, max_batch_jobs=48)
+    qtx.set_bc1_encoder(0)
     try:
         tickets = submit_all(srv, cases, outs)
         results = [srv.wait(t, 60000) for t in tickets]
