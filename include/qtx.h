@@ -229,7 +229,7 @@ typedef struct qtx_server_config {
                                budget (0: 2 GiB)                                         */
     size_t staging_bytes;   /* pinned host staging for outputs bound for pageable memory
                                (0: 256 MiB); pinned destinations are written directly   */
-    int32_t slots;          /* batches in flight at once, 1..8 (0: 3)                     */
+    int32_t slots;          /* batches in flight at once, 1..8 (0: 6)                     */
     int32_t max_batch_jobs; /* requests per batch, 1..65536 (0: 1024)                     */
     int32_t max_queue;      /* waiting requests before submit returns EBUSY (0: 65536)    */
     uint32_t unit_kops;     /* colour work-unit size, as in qtx_batch_tuning; 0: 64        */

@@ -53,6 +53,7 @@ hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uin
 constexpr uint32_t kLaneUnitBlocks = 256;
 hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits,
                              const DeviceTables *d_tables, hipStream_t stream);
-hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, hipStream_t stream);
+// grid: workgroups (at most nunits; they stride over the units)
+hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, unsigned grid, hipStream_t stream);
 
 }  // namespace qtx

@@ -445,19 +445,22 @@ __device__ __forceinline__ void decode_interp(const BatchJob &J, int gid) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict__ jobs, int njobs) {
-    const uint32_t u = blockIdx.x;
-    const BatchJob &J = unit_job(jobs, njobs, u);
-    const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
-    if (gid >= J.nblocks) return;
-    switch (J.kind) {
-        case BJ_ENC_BC4: encode_bc45_block<false>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
-        case BJ_ENC_BC5: encode_bc45_block<true>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
-        case BJ_DEC_BC1: decode_interp<DEC_BC1>(J, (int)gid); break;
-        case BJ_DEC_BC3: decode_interp<DEC_BC3>(J, (int)gid); break;
-        case BJ_DEC_BC4: decode_block<0, DEC_BC4>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
-        case BJ_DEC_BC5: decode_block<0, DEC_BC5>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
-        default: break;
+// Workgroups stride over the units, so a launch may use fewer workgroups than
+// units (a decode that writes host memory is bound by PCIe, not by the CUs).
+__global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict__ jobs, int njobs, uint32_t nunits) {
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const BatchJob &J = unit_job(jobs, njobs, u);
+        const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
+        if (gid >= J.nblocks) continue;
+        switch (J.kind) {
+            case BJ_ENC_BC4: encode_bc45_block<false>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
+            case BJ_ENC_BC5: encode_bc45_block<true>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
+            case BJ_DEC_BC1: decode_interp<DEC_BC1>(J, (int)gid); break;
+            case BJ_DEC_BC3: decode_interp<DEC_BC3>(J, (int)gid); break;
+            case BJ_DEC_BC4: decode_block<0, DEC_BC4>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
+            case BJ_DEC_BC5: decode_block<0, DEC_BC5>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
+            default: break;
+        }
     }
 }
 
@@ -624,8 +627,9 @@ hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint
     return hipGetLastError();
 }
 
-hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, hipStream_t stream) {
-    hipLaunchKernelGGL(dev::k_light_batch, dim3(nunits), dim3(kThreads), 0, stream, d_jobs, njobs);
+hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, unsigned grid, hipStream_t stream) {
+    hipLaunchKernelGGL(dev::k_light_batch, dim3(std::max(1u, std::min(grid, nunits))), dim3(kThreads), 0, stream, d_jobs,
+                       njobs, nunits);
     return hipGetLastError();
 }
 

@@ -2,16 +2,16 @@
 //
 // One loop thread per server.  Submitted requests wait in arrival order; when a
 // batch slot is free the loop admits from the front of the queue every request
-// whose input + output fit in the device arena (and, for a destination in
-// pageable memory, in the pinned staging area), stopping at the first that does
-// not fit so that no request is overtaken forever.  The admitted requests form
-// one batch (at most batch_bytes of device memory, so that several batches fit),
-// enqueued on the slot's stream:
-//     H2D of the inputs -> job table -> the batch kernels -> D2H of the outputs,
-//     smallest output first, an event after each
-// and the loop goes straight back to admitting: up to `slots` batches are on
-// the device at once, so one batch's copies overlap another's kernels and
-// requests that arrive while a batch runs join the next one.  A request whose
+// whose device footprint (a copy of a pageable input) fits in the device arena
+// and, for a destination in pageable memory, whose output fits in the pinned
+// staging area, stopping at the first that does not fit so that no request is
+// overtaken forever.  The admitted requests form
+// one batch (at most batch_bytes of traffic, so that several batches fit), run
+// as three stages on three streams -- uploads, BC1/BC3 kernels, then the decode
+// kernel and the downloads -- and the loop goes straight back to admitting: up to
+// `slots` batches are on the device at once, so one batch uploads while another
+// computes and a third downloads, and requests that arrive while a batch runs
+// join the next one.  A request whose
 // event has fired is complete and returns its memory; a slot is free again when
 // all of its requests are.
 #include <hip/hip_runtime.h>
@@ -62,23 +62,24 @@ struct Req {
     bt::Prepared prep{};
     bool dst_pinned = false;
     bool src_direct = false;  // the kernel reads the caller's pinned input in place
-    bool out_direct = false;  // the kernel writes the output into host memory (decodes)
-    size_t dev_off = 0, dev_len = 0;  // device input (unless src_direct), then device output (unless out_direct)
+    bool light = false;       // served by the decode / BC4 / BC5 kernel, else by a BC1/BC3 kernel
+    size_t dev_off = 0, dev_len = 0;  // device copy of a pageable input
     size_t stage_off = 0;
-    bool staged = false;  // the output goes through pinned staging (pageable destination)
+    bool staged = false;  // the kernel writes the output to pinned staging (pageable destination)
     bool done = false;
     qtx_server_result res{};
 };
 using ReqP = std::shared_ptr<Req>;
 
 struct Slot {
-    hipStream_t stream = nullptr;
-    std::vector<hipEvent_t> events;  // events[i]: request i's output is in its destination
+    hipEvent_t ev_in = nullptr;  // the batch's pageable inputs and job table are on the device
+    hipEvent_t ev_k = nullptr;   // its BC1/BC3 kernels are done: their outputs are in host memory
+    hipEvent_t ev_l = nullptr;   // its decode / BC4 / BC5 kernel is done
     BatchJob *d_jobs = nullptr;
     BatchJob *h_jobs = nullptr;  // pinned staging of the job table
     uint32_t *d_queues = nullptr;
-    std::vector<ReqP> reqs;  // in output-copy order
-    size_t next_done = 0;    // requests before this one are complete
+    std::vector<ReqP> reqs;
+    bool pend_k = false, pend_l = false;  // kernels of the batch not yet seen complete
     bool busy = false;
 };
 
@@ -92,6 +93,9 @@ struct qtx_server {
     uint8_t *h_stage = nullptr;
     srv::Arena arena, stage;
     std::vector<Slot> slots;
+    // the three stages of every batch: uploads; BC1/BC3 kernels; the decode /
+    // BC4 / BC5 kernel and the encode downloads
+    hipStream_t s_in = nullptr, s_k = nullptr, s_out = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ReqP> waiting;
@@ -112,57 +116,56 @@ struct qtx_server {
 
 namespace {
 
-// Device memory a request needs: its input unless the kernel reads it in place,
-// its output unless the kernel writes it to host memory.
-size_t footprint(const Req &r) {
-    return (r.src_direct ? 0 : align_up(r.prep.src_bytes)) + (r.out_direct ? 0 : align_up(r.prep.dst_bytes));
-}
+// Device memory a request needs: a copy of its input when that is pageable (a
+// pinned input is read in place; every output is written straight into host memory).
+size_t footprint(const Req &r) { return r.src_direct ? 0 : align_up(r.prep.src_bytes); }
 
-// Enqueue one batch on a slot (called without the lock; touches only the
-// slot and the requests, which the loop owns while they are admitted).  reqs
-// holds the decodes first, then the encodes by ascending output size.
-//   decodes: blocks are read from the caller's pinned buffer in place (or
-//     copied in when pageable) and the kernel stores texels straight into host
-//     memory -- the caller's pinned buffer or the staging area -- so the large
-//     direction of a decode never queues behind the copy engine;
-//   encodes: input copied in, blocks copied out, smallest output first.
-// events[i] fires when request i's output is in host memory.
+// Enqueue one batch (called without the lock; touches only the slot, the
+// requests, which the loop owns while they are admitted, and the stage streams,
+// which only the loop uses).  Stages:
+//   s_in:  pageable inputs and the job table
+//   s_k:   the BC1/BC3 kernels (after s_in)
+//   s_out: the decode / BC4 / BC5 kernel (after s_in)
+// Kernels read pinned inputs in place over PCIe and store every output straight
+// into host memory -- the caller's pinned buffer or the staging area -- so a batch
+// costs one table upload and its kernels, whatever its size; PCIe reads and writes
+// of different batches and the copy engine's uploads overlap.
 hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
-    hipStream_t st = sl.stream;
     std::vector<bt::Prepared> prep(reqs.size());
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
         const Req &r = *reqs[i];
         prep[i] = r.prep;
-        uint8_t *dev = s->d_arena + r.dev_off;
-        const size_t in_len = r.src_direct ? 0 : align_up(r.prep.src_bytes);
         if (r.src_direct) {
             prep[i].job.src = static_cast<const uint8_t *>(r.job.src);
         } else {
+            uint8_t *dev = s->d_arena + r.dev_off;
             prep[i].job.src = dev;
-            e = hipMemcpyAsync(dev, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(dev, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, s->s_in);
         }
-        if (r.out_direct)
-            prep[i].job.dst = r.staged ? s->h_stage + r.stage_off : static_cast<uint8_t *>(r.job.dst);
-        else
-            prep[i].job.dst = dev + in_len;
+        prep[i].job.dst = r.staged ? s->h_stage + r.stage_off : static_cast<uint8_t *>(r.job.dst);
     }
     if (e != hipSuccess) return e;
     bt::Plan plan;
     const qtx_batch_tuning tuning{bt::kMaxColourWgPerCu, s->cfg.unit_kops};
     if (bt::plan(prep, tuning, s->ctx->cu_count, s->bc1_path, plan) != QTX_OK) return hipErrorInvalidValue;
     std::memcpy(sl.h_jobs, plan.host.data(), plan.host.size() * sizeof(BatchJob));
-    e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st);
-    for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++)
-        e = bt::enqueue_group(plan.groups[g], sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
-    for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
-        const Req &r = *reqs[i];
-        if (!r.out_direct) {
-            void *dst = r.staged ? (void *)(s->h_stage + r.stage_off) : r.job.dst;
-            e = hipMemcpyAsync(dst, prep[i].job.dst, r.prep.dst_bytes, hipMemcpyDeviceToHost, st);
-        }
-        if (e == hipSuccess) e = hipEventRecord(sl.events[i], st);
+    e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s->s_in);
+    if (e == hipSuccess) e = hipEventRecord(sl.ev_in, s->s_in);
+    sl.pend_k = sl.pend_l = false;
+    for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++) {
+        bt::Plan::Group grp = plan.groups[g];
+        const bool light = grp.kind == bt::kGroupLight;
+        hipStream_t st = light ? s->s_out : s->s_k;
+        bool &pend = light ? sl.pend_l : sl.pend_k;
+        if (!pend) e = hipStreamWaitEvent(st, sl.ev_in, 0);
+        pend = true;
+        // PCIe-bound: leave the CUs to the encoders
+        if (light) grp.grid = std::min<unsigned>(grp.grid, 2u * (unsigned)s->ctx->cu_count);
+        if (e == hipSuccess) e = bt::enqueue_group(grp, sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
     }
+    if (e == hipSuccess && sl.pend_k) e = hipEventRecord(sl.ev_k, s->s_k);
+    if (e == hipSuccess && sl.pend_l) e = hipEventRecord(sl.ev_l, s->s_out);
     return e;
 }
 
@@ -197,42 +200,40 @@ void run(qtx_server *s) {
     (void)hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        // 1. completed requests, in each slot's output-copy order
+        // 1. completed kernels: their requests' outputs are in host memory
         for (Slot &sl : s->slots) {
             if (!sl.busy) continue;
-            const size_t done_before = sl.next_done;
-            while (sl.next_done < sl.reqs.size()) {
-                const hipError_t q = hipEventQuery(sl.events[sl.next_done]);
-                if (q == hipErrorNotReady) break;
+            bool progressed = false;
+            for (int which = 0; which < 2; which++) {
+                bool &pend = which ? sl.pend_l : sl.pend_k;
+                if (!pend) continue;
+                const hipError_t q = hipEventQuery(which ? sl.ev_l : sl.ev_k);
+                if (q == hipErrorNotReady) continue;
+                pend = false;
+                progressed = true;
+                std::string msg;
                 if (q != hipSuccess) {
-                    const std::string msg = std::string("batch failed: ") + hipGetErrorString(q);
+                    msg = std::string("batch failed: ") + hipGetErrorString(q);
                     s->failed = true;
                     s->fail_msg = msg;
                     lk.unlock();
-                    (void)hipStreamSynchronize(sl.stream);
+                    for (hipStream_t st : {s->s_in, s->s_k, s->s_out}) (void)hipStreamSynchronize(st);
                     lk.lock();
-                    const double t = s->now();
-                    for (size_t i = sl.next_done; i < sl.reqs.size(); i++) {
-                        release(s, *sl.reqs[i]);
-                        finish(s, *sl.reqs[i], QTX_EDEVICE, msg.c_str(), t);
+                }
+                for (auto &r : sl.reqs) {
+                    if (r->light != (which == 1) || r->done) continue;
+                    if (q == hipSuccess && r->staged) {
+                        lk.unlock();
+                        std::memcpy(r->job.dst, s->h_stage + r->stage_off, r->prep.dst_bytes);
+                        lk.lock();
                     }
-                    sl.next_done = sl.reqs.size();
-                    break;
+                    release(s, *r);
+                    finish(s, *r, q == hipSuccess ? QTX_OK : QTX_EDEVICE, q == hipSuccess ? nullptr : msg.c_str(),
+                           s->now());
                 }
-                ReqP r = sl.reqs[sl.next_done];
-                if (r->staged) {
-                    lk.unlock();
-                    std::memcpy(r->job.dst, s->h_stage + r->stage_off, r->prep.dst_bytes);
-                    lk.lock();
-                }
-                release(s, *r);
-                finish(s, *r, QTX_OK, nullptr, s->now());
-                sl.next_done++;
             }
-            const bool progressed = sl.next_done != done_before;
-            if (sl.next_done == sl.reqs.size()) {
+            if (!sl.pend_k && !sl.pend_l) {
                 sl.reqs.clear();
-                sl.next_done = 0;
                 sl.busy = false;
                 update_busy(s, s->now());
             }
@@ -251,10 +252,9 @@ void run(qtx_server *s) {
             auto reserve = [&](ReqP &r) {
                 size_t off = 0, soff = 0;
                 const size_t need = footprint(*r);
-                // a batch holds at most batch_bytes (a larger request goes alone), so
-                // several batches fit in the arena and their copies and kernels overlap;
-                // bytes the kernels move over PCIe count against the same limit
-                const size_t moved = need + (r->out_direct ? r->prep.dst_bytes : 0) + (r->src_direct ? r->prep.src_bytes : 0);
+                // a batch moves at most batch_bytes (a larger request goes alone), so
+                // several batches are in flight and their transfers and kernels overlap
+                const size_t moved = r->prep.src_bytes + r->prep.dst_bytes;
                 if (batch_host_bytes > 0 && batch_host_bytes + moved > s->cfg.batch_bytes) return false;
                 if (need && !s->arena.alloc(need, off)) return false;
                 if (!r->dst_pinned) {
@@ -273,15 +273,6 @@ void run(qtx_server *s) {
             srv::admit_fifo(s->waiting, (size_t)s->cfg.max_batch_jobs, reserve, taken);
             if (!taken.empty()) {
                 std::vector<ReqP> reqs(taken.begin(), taken.end());
-                std::stable_sort(reqs.begin(), reqs.end(), [](const ReqP &x, const ReqP &y) {
-                    if (x->out_direct != y->out_direct) return x->out_direct;  // decodes first
-                    return x->prep.dst_bytes < y->prep.dst_bytes;
-                });
-                while (free_slot->events.size() < reqs.size()) {
-                    hipEvent_t ev = nullptr;
-                    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) break;
-                    free_slot->events.push_back(ev);
-                }
                 const double t_admit = s->now();
                 for (auto &r : reqs) {
                     r->res.admit_s = t_admit;
@@ -293,13 +284,14 @@ void run(qtx_server *s) {
                 free_slot->busy = true;
                 update_busy(s, t_admit);
                 lk.unlock();
-                const hipError_t e = free_slot->events.size() < reqs.size() ? hipErrorOutOfMemory
-                                                                            : enqueue(s, *free_slot, reqs);
+                const hipError_t e = enqueue(s, *free_slot, reqs);
                 lk.lock();
                 const double t_launch = s->now();
                 if (e != hipSuccess) {
                     // nothing of this batch can be trusted; wait for what was enqueued
-                    (void)hipStreamSynchronize(free_slot->stream);
+                    (void)hipStreamSynchronize(s->s_in);
+                    (void)hipStreamSynchronize(s->s_k);
+                    (void)hipStreamSynchronize(s->s_out);
                     const std::string msg = std::string("batch enqueue failed: ") + hipGetErrorString(e);
                     s->failed = true;
                     s->fail_msg = msg;
@@ -313,7 +305,6 @@ void run(qtx_server *s) {
                 } else {
                     for (auto &r : reqs) r->res.launch_s = t_launch;
                     free_slot->reqs = std::move(reqs);
-                    free_slot->next_done = 0;
                 }
                 continue;  // look for completions and another free slot at once
             }
@@ -336,14 +327,17 @@ void run(qtx_server *s) {
 
 void destroy(qtx_server *s) {
     (void)hipSetDevice(s->ctx->device);
+    for (hipStream_t st : {s->s_in, s->s_k, s->s_out})
+        if (st) (void)hipStreamSynchronize(st);
     for (Slot &sl : s->slots) {
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        for (hipEvent_t ev : sl.events) (void)hipEventDestroy(ev);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        for (hipEvent_t ev : {sl.ev_in, sl.ev_k, sl.ev_l})
+            if (ev) (void)hipEventDestroy(ev);
         if (sl.d_jobs) (void)hipFree(sl.d_jobs);
         if (sl.d_queues) (void)hipFree(sl.d_queues);
         if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
     }
+    for (hipStream_t st : {s->s_in, s->s_k, s->s_out})
+        if (st) (void)hipStreamDestroy(st);
     if (s->d_arena) (void)hipFree(s->d_arena);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     delete s;
@@ -362,7 +356,7 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     if (config) c = *config;
     if (!c.device_bytes) c.device_bytes = (size_t)2 << 30;
     if (!c.staging_bytes) c.staging_bytes = (size_t)256 << 20;
-    if (!c.slots) c.slots = 3;
+    if (!c.slots) c.slots = 6;
     if (!c.max_batch_jobs) c.max_batch_jobs = 1024;
     if (!c.max_queue) c.max_queue = 65536;
     if (!c.unit_kops) c.unit_kops = 64;
@@ -380,9 +374,14 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     s->origin = std::chrono::steady_clock::now();
     hipError_t e = hipMalloc(&s->d_arena, c.device_bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_stage, c.staging_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_in, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_out, hipStreamNonBlocking);
     s->slots.resize((size_t)c.slots);
     for (Slot &sl : s->slots) {
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_k, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_l, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob));
         if (e == hipSuccess)
             e = hipHostMalloc((void **)&sl.h_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob), hipHostMallocDefault);
@@ -412,10 +411,8 @@ qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes
         return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", dst_bytes, r->prep.dst_bytes);
     (void)hipSetDevice(s->ctx->device);
     r->dst_pinned = is_pinned(job->dst);
-    if (job->op == QTX_OP_DECODE) {
-        r->out_direct = true;
-        r->src_direct = is_pinned(job->src);
-    }
+    r->src_direct = is_pinned(job->src);
+    r->light = r->prep.group == bt::kGroupLight;
     const size_t need = footprint(*r);
     if (need > s->arena.capacity())
         return fail(QTX_ENOMEM, "request needs %zu bytes of device memory; the server has %zu", need,
