@@ -225,8 +225,9 @@ qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *d
  * (quicktex/Encoder.h:36-56, quicktex/Decoder.h:36-54) for callers with
  * many independent requests. */
 typedef struct qtx_server_config {
-    size_t device_bytes;    /* HBM reserved for request inputs + outputs, the admission
-                               budget (0: 2 GiB)                                         */
+    size_t device_bytes;    /* HBM arena for copies of pageable request inputs, the
+                               admission budget (0: 2 GiB); pinned inputs are read in
+                               place and outputs are written to host memory directly     */
     size_t staging_bytes;   /* pinned host staging for outputs bound for pageable memory
                                (0: 256 MiB); pinned destinations are written directly   */
     int32_t slots;          /* batches in flight at once, 1..8 (0: 6)                     */
