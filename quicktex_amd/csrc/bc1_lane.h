@@ -35,6 +35,7 @@ struct LaneLds {
     uint32_t match[2][256];  // this launch's interpolator, 5- and 6-bit
     float mid5[32], mid6[64];
     uint2 sums[16][kThreads];  // sums[k][t]: prefix sum k+1 of thread t's sorted texels (r | g << 16, b)
+    uint32_t wave_state[kThreads / 64][4];  // the queued kernel's per-wave dealing state
 };
 __device__ __forceinline__ LaneLds &LL() { return *reinterpret_cast<LaneLds *>(qtx_lds_raw); }
 

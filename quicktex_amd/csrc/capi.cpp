@@ -51,6 +51,37 @@ size_t block_bytes(int32_t w, int32_t h, int bytes_per_block) {
 
 size_t raw_bytes(int32_t w, int32_t h) { return (size_t)w * (size_t)h * 4; }
 
+QueueLease::QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted) : ctx_(ctx), stream_(stream) {
+    if (!wanted) return;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
+    ctx->queue_mu.lock();
+    locked_ = true;
+    for (int i = 0; i < qtx_ctx::kQueueSlots; i++) {
+        hipEvent_t &ev = ctx->queue_done[i];
+        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+            break;
+        }
+        if (hipEventQuery(ev) == hipSuccess) {  // never recorded, or its launch is done
+            slot_ = i;
+            words = ctx->d_queues + (size_t)kLaneQueueWords * i;
+            return;
+        }
+    }
+}
+
+hipError_t QueueLease::commit() {
+    hipError_t e = hipSuccess;
+    if (slot_ >= 0) e = hipEventRecord(ctx_->queue_done[slot_], stream_);
+    slot_ = -1;
+    if (locked_) ctx_->queue_mu.unlock();
+    locked_ = false;
+    return e;
+}
+
+QueueLease::~QueueLease() { (void)commit(); }
+
 qtx_status check_ctx(qtx_ctx *ctx) {
     if (!ctx) return fail(QTX_EINVAL, "null context");
     QTX_HIP(hipSetDevice(ctx->device));
@@ -179,6 +210,9 @@ qtx_status qtx_ctx_create(int32_t device, qtx_ctx **out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->d_tables, sizeof(qtx::DeviceTables));
     if (e == hipSuccess)
         e = hipMemcpy(ctx->d_tables, &qtx::host_tables(), sizeof(qtx::DeviceTables), hipMemcpyHostToDevice);
+    const size_t qbytes = sizeof(uint32_t) * qtx::kLaneQueueWords * qtx_ctx::kQueueSlots;
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_queues, qbytes);
+    if (e == hipSuccess) e = hipMemset(ctx->d_queues, 0, qbytes);
     if (e != hipSuccess) {
         qtx_ctx_destroy(ctx);
         return hip_fail(e, "context set-up");
@@ -194,6 +228,12 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
     if (ctx->d_in) (void)hipFree(ctx->d_in);
     if (ctx->d_out) (void)hipFree(ctx->d_out);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    for (hipEvent_t &ev : ctx->queue_done) {
+        if (!ev) continue;
+        (void)hipEventSynchronize(ev);  // the last launch that used the slot
+        (void)hipEventDestroy(ev);
+    }
+    if (ctx->d_queues) (void)hipFree(ctx->d_queues);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QTX_OK;
@@ -226,8 +266,10 @@ qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     int interp = 0;
     if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QueueLease lease(ctx, (hipStream_t)stream, qtx::wants_lane_queue(s, width, height, ctx->bc1_path));
     QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false, ctx->bc1_path,
-                                   (hipStream_t)stream));
+                                   lease.words, ctx->cu_count, (hipStream_t)stream));
+    QTX_HIP(lease.commit());
     return QTX_OK;
 }
 
@@ -241,8 +283,10 @@ qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
     if (s.color_mode != QTX_COLOR_FOUR) return fail(QTX_EINVAL, "BC3 colour blocks are always FourColor");
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    QueueLease lease(ctx, (hipStream_t)stream, qtx::wants_lane_queue(s, width, height, ctx->bc1_path));
     QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, true, ctx->bc1_path,
-                                   (hipStream_t)stream));
+                                   lease.words, ctx->cu_count, (hipStream_t)stream));
+    QTX_HIP(lease.commit());
     return QTX_OK;
 }
 

@@ -21,6 +21,15 @@ inline int bc1_path_env() {
     }();
     return p;
 }
+// QTX_LANE_QUEUE=0 in the environment: the lane encoder keeps one block per lane
+// and takes no work queue (for comparisons; the bytes are the same).
+inline bool lane_queue_env() {
+    static const bool on = [] {
+        const char *e = std::getenv("QTX_LANE_QUEUE");
+        return !(e && !std::strcmp(e, "0"));
+    }();
+    return on;
+}
 // Estimated ALU ops per BC1 block for the encoder settings.
 double bc1_block_ops(const BC1Settings &s);
 // Whether the lane-per-block encoder serves settings s when `blocks` blocks of
@@ -32,8 +41,24 @@ constexpr uint64_t kLaneMinBlocks = 32768;
 constexpr double kLaneMaxOps = 120000.0;
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path);
 
+// Work-queue words of the queued lane encoder: kLaneQueues block counters, each
+// on its own 64-byte line, and a count of finished waves.
+constexpr int kLaneQueues = 8;
+constexpr int kLaneQueueStride = 16;
+constexpr int kLaneRetireWord = kLaneQueues * kLaneQueueStride;
+constexpr int kLaneQueueWords = 256;  // per queue, rounded up (1 KiB)
+// The queued kernel serves settings from this estimated cost per block up; on
+// cheaper settings its refills (an atomic and a load per round) outweigh the
+// lanes it saves.
+constexpr double kLaneQueueMinOps = 10000.0;
+// Whether launch_bc1_encode would use queue words for this texture (so that the
+// caller need not lend a set otherwise).
+bool wants_lane_queue(const BC1Settings &s, int w, int h, int path);
+// d_queue: kLaneQueueWords zeroed words that no other launch uses until this one
+// has completed (the lane encoder then deals blocks from queues), or nullptr.
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, int path, hipStream_t stream);
+                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, int cu_count,
+                             hipStream_t stream);
 hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, int ch0, int ch1, bool bc5,
                              hipStream_t stream);
 // format: QTX_FMT_*; p0/p1: write_alpha for BC1, channels for BC4/BC5

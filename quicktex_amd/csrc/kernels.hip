@@ -139,14 +139,10 @@ __device__ __forceinline__ void load_block16(const uint8_t *__restrict__ rgba, i
     }
 }
 
-// One BC1 (or BC3) block per lane: encode and store.
-template <int INTERP>
-__device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
-                                                  uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
-                                                  const BC1Settings &s, bool bc3) {
-    uint32_t px[16];
-    load_block16(rgba, w, h, bw, gid, px);
-    const uint2 colour = lane::encode_bc1<INTERP>(px, s, T);
+// Stores block gid: the colour block, and for BC3 the alpha block of its texels
+// before it.
+__device__ __forceinline__ void store_lane_block(uint8_t *__restrict__ out, uint32_t gid, const uint32_t (&px)[16],
+                                                 uint2 colour, bool bc3) {
     if (bc3) {
         uint32_t a[16];
 #pragma unroll
@@ -158,6 +154,16 @@ __device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rg
     }
 }
 
+// One BC1 (or BC3) block per lane: encode and store.
+template <int INTERP>
+__device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+                                                  uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                  const BC1Settings &s, bool bc3) {
+    uint32_t px[16];
+    load_block16(rgba, w, h, bw, gid, px);
+    store_lane_block(out, (uint32_t)gid, px, lane::encode_bc1<INTERP>(px, s, T), bc3);
+}
+
 // BC1/BC3 encode, one block per lane, for settings without endpoint search.
 template <int INTERP, bool BC3>
 __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
@@ -167,6 +173,118 @@ __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__
     __syncthreads();
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (gid < nblocks) encode_lane_block<INTERP>(rgba, w, h, bw, gid, out, T, s, BC3);
+}
+
+// queue[0] hands out units and queue[1] counts the waves that found it empty;
+// the last of those waves returns both to zero, so the next launch of the same
+// kernel on the same queue (stream-ordered after this one) starts from zero with
+// no fill or memset in between.  Every wave calls this exactly once.
+__device__ __forceinline__ void retire(uint32_t *queue, uint32_t waves) {
+    if ((threadIdx.x & 63) == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == waves - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// The same encoder with the blocks dealt from queues, so that the searches run
+// on full waves.  A solid block (one RGBA value: EncodeBlock writes it from the
+// single-colour tables, BC1Encoder.cpp:262-268) costs a few dozen instructions,
+// every other block tens of thousands; with one fixed block per lane, each solid
+// block would leave its lane idle while the wave searches the others.  Here each
+// lane holds a non-solid block before the wave starts a search: the lanes without
+// one take the next block indices of the wave's current range (one atomic per
+// wave and refill, indices ranked among the lanes that asked), store a solid block
+// on the spot and ask again.
+//
+// The blocks are split into kLaneQueues contiguous ranges with one counter each
+// (one address sustains only so many atomics per second); a wave starts on the
+// range of its workgroup's XCD (workgroups go to the 8 XCDs round-robin) and moves
+// on to the next range when its own is used up.  A wave claims exactly the blocks
+// it places, so no blocks sit reserved by a wave when the ranges run dry: a search
+// runs on fewer than 64 lanes only at the very end.  The grid is persistent (at
+// most the resident workgroups).  The queue words (kLaneQueueWords, zero at
+// launch) are left zero by the last wave to finish, so launches on one queue must
+// be stream-ordered.
+__device__ __forceinline__ void retire_lane_queue(uint32_t *queue, uint32_t waves) {
+    if ((threadIdx.x & 63) == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[kLaneRetireWord], 1u) == waves - 1) {
+            for (int q = 0; q < kLaneQueues; q++) atomicExch(&queue[q * kLaneQueueStride], 0u);
+            atomicExch(&queue[kLaneRetireWord], 0u);
+        }
+    }
+}
+
+template <int INTERP, bool BC3>
+__global__ __launch_bounds__(256, 3) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
+                                                        uint32_t nblocks, uint8_t *__restrict__ out,
+                                                        const DeviceTables *__restrict__ T, BC1Settings s,
+                                                        uint32_t *__restrict__ queue) {
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    lane::load_tables<INTERP>(T);
+    // the wave's current range and the number of ranges it found used up live in
+    // LDS between refills, so that they hold no registers through the searches
+    uint32_t *const ws = lane::LL().wave_state[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+    if ((threadIdx.x & 63) == 0) {
+        ws[0] = blockIdx.x % kLaneQueues;
+        ws[1] = 0;
+    }
+    __syncthreads();
+    uint32_t px[16];
+    bool open = true;  // wave-uniform: some range may still hold blocks
+    for (;;) {
+        uint32_t gid = kNone;
+        if (open) {
+            const uint32_t per = (nblocks + kLaneQueues - 1) / kLaneQueues;  // blocks per range
+            uint32_t q = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws[0]);
+            uint32_t exhausted = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws[1]);
+            // refill until every lane holds a non-solid block or every range is used up
+            for (;;) {
+                const uint64_t need = __ballot(gid == kNone);
+                if (need == 0) break;
+                if (exhausted == kLaneQueues) {
+                    open = false;
+                    break;
+                }
+                const uint32_t want = (uint32_t)__popcll(need);
+                uint32_t base = 0;
+                if ((threadIdx.x & 63) == 0) base = atomicAdd(&queue[q * kLaneQueueStride], want);
+                base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+                const uint32_t begin = q * per, end = min(begin + per, nblocks);
+                const uint32_t size = end > begin ? end - begin : 0u;
+                const uint32_t got = base < size ? min(want, size - base) : 0u;
+                if (gid == kNone) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+                    if (rank < got) {
+                        const uint32_t idx = begin + base + rank;
+                        load_block16(rgba, w, h, bw, (int)idx, px);
+                        bool solid = true;
+#pragma unroll
+                        for (int i = 1; i < 16; i++) solid &= px[i] == px[0];
+                        if (solid) store_lane_block(out, idx, px, lane::write_solid(px[0]), BC3);
+                        else gid = idx;
+                    }
+                }
+                if (got < want) {  // this range is used up: continue on the next one
+                    q = (q + 1) % kLaneQueues;
+                    exhausted++;
+                }
+            }
+            if ((threadIdx.x & 63) == 0) {
+                ws[0] = q;
+                ws[1] = exhausted;
+            }
+            // keep the state in LDS: no value of it is carried in registers past here
+            asm volatile("" ::: "memory");
+        }
+        if (__ballot(gid != kNone) == 0) break;
+        if (gid != kNone) store_lane_block(out, gid, px, lane::encode_bc1<INTERP>(px, s, T), BC3);
+    }
+    retire_lane_queue(queue, gridDim.x * (blockDim.x / 64));
 }
 
 // BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
@@ -351,19 +469,6 @@ __device__ __forceinline__ uint32_t take_unit(uint32_t *queue) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)u);
 }
 
-// queue[0] hands out units and queue[1] counts the waves that found it empty;
-// the last of those waves returns both to zero, so the next launch of the same
-// kernel on the same queue (stream-ordered after this one) starts from zero with
-// no fill or memset in between.  Every wave calls this exactly once.
-__device__ __forceinline__ void retire(uint32_t *queue, uint32_t waves) {
-    if ((threadIdx.x & 63) == 0) {
-        __threadfence();
-        if (atomicAdd(&queue[1], 1u) == waves - 1) {
-            atomicExch(&queue[0], 0u);
-            atomicExch(&queue[1], 0u);
-        }
-    }
-}
 
 // Persistent BC1/BC3 encoder over the jobs of one interpolator.  Each wave takes
 // a unit (unit_blocks consecutive blocks of one job), loads the job's settings
@@ -501,12 +606,42 @@ bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
     return blocks >= kLaneMinBlocks && bc1_block_ops(s) <= kLaneMaxOps;
 }
 
+bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
+    const uint64_t nblocks = (uint64_t)((w + 3) / 4) * (uint64_t)((h + 3) / 4);
+    return use_lane_encoder(s, nblocks, path) && lane_queue_env() && bc1_block_ops(s) >= kLaneQueueMinOps;
+}
+
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, int path, hipStream_t stream) {
+                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, int cu_count,
+                             hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
     if (use_lane_encoder(s, (uint64_t)nblocks, path)) {
-        const dim3 lgrid(grid_for(nblocks, kThreads)), lblock(kThreads);
+        const dim3 lblock(kThreads);
+        if (d_queue && wants_lane_queue(s, w, h, path)) {
+            // persistent: the resident workgroups (3 per CU), fewer for small launches
+            const long long resident = 3LL * std::max(cu_count, 1);
+            const dim3 qgrid((unsigned)std::min<long long>(grid_for(nblocks, kThreads), resident));
+#define QTX_LANE_CASE(I)                                                                                            \
+    case I:                                                                                                         \
+        if (bc3)                                                                                                    \
+            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, true>), qgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, \
+                               (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                     \
+        else                                                                                                        \
+            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, false>), qgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw,\
+                               (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                     \
+        break;
+            switch (interp) {
+                QTX_LANE_CASE(QTX_INTERP_IDEAL)
+                QTX_LANE_CASE(QTX_INTERP_IDEAL_ROUND)
+                QTX_LANE_CASE(QTX_INTERP_NVIDIA)
+                QTX_LANE_CASE(QTX_INTERP_AMD)
+                default: return hipErrorInvalidValue;
+            }
+#undef QTX_LANE_CASE
+            return hipGetLastError();
+        }
+        const dim3 lgrid(grid_for(nblocks, kThreads));
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
         if (bc3)                                                                                                    \

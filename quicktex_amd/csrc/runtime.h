@@ -23,6 +23,14 @@ struct qtx_ctx {
     uint8_t *d_out = nullptr;
     size_t out_cap = 0;
     std::mutex mu;
+    // Work-queue words of the queued BC1/BC3 lane kernel: kQueueSlots sets, each
+    // lent to one launch at a time.  A set is free again once the event recorded
+    // after its last launch has completed (the kernel leaves its words zero), so
+    // concurrent launches -- on any streams -- never share one.
+    static constexpr int kQueueSlots = 64;
+    uint32_t *d_queues = nullptr;
+    hipEvent_t queue_done[kQueueSlots] = {};  // created on a slot's first use
+    std::mutex queue_mu;
 };
 
 namespace qtx {
@@ -40,6 +48,24 @@ qtx_status to_settings(const qtx_bc1_params *p, BC1Settings &s, int &interp);
 // grow a device buffer to at least need bytes (contents are not kept)
 qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need);
 size_t block_bytes(int32_t w, int32_t h, int bytes_per_block);
+// A set of the context's queue words lent to one launch on `stream`: `words`
+// is nullptr when every set is in use (or the stream is capturing a graph) and
+// the caller launches the queue-less kernel.  commit() marks the set busy until
+// the work enqueued on the stream so far has completed.  Holds the context's
+// queue lock from construction to commit() or destruction.
+class QueueLease {
+  public:
+    QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted);
+    ~QueueLease();
+    uint32_t *words = nullptr;
+    hipError_t commit();
+
+  private:
+    qtx_ctx *ctx_;
+    hipStream_t stream_;
+    int slot_ = -1;
+    bool locked_ = false;
+};
 size_t raw_bytes(int32_t w, int32_t h);
 
 }  // namespace rt
