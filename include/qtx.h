@@ -215,6 +215,38 @@ qtx_status qtx_batch_run(qtx_ctx *ctx, const qtx_job *jobs, size_t njobs, void *
 qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *dst_bytes, size_t njobs);
 
 /* ---------------------------------------------------------------------- */
+/* Mip chains: the levels quicktex.dds.encode writes (quicktex/dds.py:276-287).
+ * Level sizes follow quicktex.image_utils.mip_sizes (image_utils.py:9-38): each
+ * level halves both sides of the one before (rounding down, at least 1) until
+ * 1x1.  Every level is resampled from the base (level 0) as
+ * image_utils.resize_no_premultiply does (image_utils.py:42-52): Pillow's
+ * Image.BILINEAR resize of each RGBA8 channel, bit-exact. */
+
+/* Levels in the chain of a width x height texture: the full chain for
+ * requested <= 0, else min(requested, full) -- mip_sizes(size, mip_count). */
+int32_t qtx_mip_count(int32_t width, int32_t height, int32_t requested);
+/* Device workspace qtx_mips_device needs for a chain of `levels` levels (0 when
+ * the request is invalid). */
+size_t qtx_mips_workspace_bytes(int32_t width, int32_t height, int32_t levels);
+/* Resamples levels 1 .. levels-1 of the chain of the RGBA8 texture d_rgba into
+ * d_levels[0 .. levels-2] (device memory, RGBA8 of each level's size), on
+ * stream.  d_workspace: qtx_mips_workspace_bytes bytes of device memory, not
+ * used by anything else until the work completes. */
+qtx_status qtx_mips_device(qtx_ctx *ctx, const uint8_t *d_rgba, int32_t width, int32_t height, int32_t levels,
+                           uint8_t *const *d_levels, void *d_workspace, size_t workspace_bytes, void *stream);
+/* Host-memory form: out_levels[m-1] (capacity out_bytes[m-1]) receives level m. */
+qtx_status qtx_mips_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, int32_t height, int32_t levels,
+                         uint8_t *const *out_levels, const size_t *out_bytes);
+/* The encoder side of quicktex.dds.encode: resamples the chain on the device and
+ * encodes every level (the base included) as one batch, the levels never
+ * leaving device memory.  tmpl: an encode job whose op, format, bc1 params and
+ * channels apply to every level (its size and buffers are ignored);
+ * out_blocks[m] (capacity out_bytes[m]) receives level m's blocks, byte-identical
+ * to encoding each level with the plain entry points. */
+qtx_status qtx_mips_encode_host(qtx_ctx *ctx, const qtx_job *tmpl, const uint8_t *rgba, int32_t width, int32_t height,
+                                int32_t levels, uint8_t *const *out_blocks, const size_t *out_bytes);
+
+/* ---------------------------------------------------------------------- */
 /* Serving form: a request loop on its own thread.  Requests (encode or
  * decode, any format, size and settings; host buffers) are admitted in
  * arrival order as device memory frees up, grouped into batches -- every

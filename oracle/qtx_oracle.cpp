@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "../include/qtx.h"
 #include "../quicktex_amd/csrc/order_tables.inc"
@@ -1263,3 +1264,110 @@ int qtxo_midpoints(float *out5 /* 32 */, float *out6 /* 64 */) {
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------------------
+// Mip levels of quicktex.dds.encode (quicktex/dds.py:280): every level is
+// resampled from the base image by image_utils.resize_no_premultiply
+// (quicktex/image_utils.py:42-52), i.e. Pillow's Image.resize with
+// Image.BILINEAR on the RGB bands and, separately, on the alpha band.  Pillow
+// (a dependency of the reference, unpinned in pyproject.toml:30; 12.2.0 in this
+// image) implements it in libImaging/Resample.c; restated here from its
+// published algorithm:
+//   * per axis, coefficients of the triangle filter (support 1) scaled by the
+//     downscale factor, centred at in0 + (i + 0.5) * scale, each window
+//     [xmin, xmin + n) clipped to the image and normalised to sum 1, in double;
+//   * the weights as fixed point with 22 fraction bits, rounded half away
+//     from zero;
+//   * a horizontal pass into an 8-bit image, then a vertical pass, each output
+//     = clamp((2^21 + sum(v * k)) >> 22, 0, 255); a pass is skipped when that
+//     axis keeps its size.
+// Bands are resampled independently with the same coefficients, so resampling
+// all four channels of RGBA8 equals resize_no_premultiply on RGBA and
+// Image.resize on RGBX.  Pinned against Pillow itself in tests/test_oracle.py.
+namespace {
+
+constexpr int kResampleBits = 22;
+
+struct Coeffs {
+    int ksize = 0;
+    std::vector<int32_t> k;      // out x ksize
+    std::vector<int32_t> lo, n;  // window start and length per output
+};
+
+Coeffs resample_coeffs(int in_size, int out_size) {
+    Coeffs c;
+    const double scale = (double)in_size / (double)out_size;
+    const double filterscale = scale < 1.0 ? 1.0 : scale;
+    const double support = 1.0 * filterscale;
+    c.ksize = (int)std::ceil(support) * 2 + 1;
+    c.k.assign((size_t)out_size * c.ksize, 0);
+    c.lo.resize(out_size);
+    c.n.resize(out_size);
+    std::vector<double> w(c.ksize);
+    for (int xx = 0; xx < out_size; xx++) {
+        const double center = (xx + 0.5) * scale;
+        const double ss = 1.0 / filterscale;
+        int xmin = (int)(center - support + 0.5);
+        if (xmin < 0) xmin = 0;
+        int xmax = (int)(center + support + 0.5);
+        if (xmax > in_size) xmax = in_size;
+        xmax -= xmin;
+        double ww = 0.0;
+        for (int x = 0; x < xmax; x++) {
+            double t = (x + xmin - center + 0.5) * ss;
+            if (t < 0.0) t = -t;
+            w[x] = t < 1.0 ? 1.0 - t : 0.0;
+            ww += w[x];
+        }
+        for (int x = 0; x < xmax; x++) {
+            const double v = ww != 0.0 ? w[x] / ww : w[x];
+            c.k[(size_t)xx * c.ksize + x] =
+                v < 0 ? (int32_t)(-0.5 + v * (1 << kResampleBits)) : (int32_t)(0.5 + v * (1 << kResampleBits));
+        }
+        c.lo[xx] = xmin;
+        c.n[xx] = xmax;
+    }
+    return c;
+}
+
+inline uint8_t clip8(int32_t v) {
+    v >>= kResampleBits;
+    return (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+}
+
+}  // namespace
+
+extern "C" int qtxo_resize(const uint8_t *rgba, int32_t w, int32_t h, uint8_t *out, int32_t ow, int32_t oh) {
+    if (w <= 0 || h <= 0 || ow <= 0 || oh <= 0) return fail("bad size");
+    std::vector<uint8_t> tmp;
+    const uint8_t *src = rgba;
+    int sw = w;
+    if (ow != w) {  // horizontal pass
+        const Coeffs c = resample_coeffs(w, ow);
+        tmp.resize((size_t)ow * h * 4);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < ow; x++)
+                for (int ch = 0; ch < 4; ch++) {
+                    int32_t ss = 1 << (kResampleBits - 1);
+                    for (int t = 0; t < c.n[x]; t++)
+                        ss += rgba[((size_t)y * w + c.lo[x] + t) * 4 + ch] * c.k[(size_t)x * c.ksize + t];
+                    tmp[((size_t)y * ow + x) * 4 + ch] = clip8(ss);
+                }
+        src = tmp.data();
+        sw = ow;
+    }
+    if (oh != h) {  // vertical pass
+        const Coeffs c = resample_coeffs(h, oh);
+        for (int y = 0; y < oh; y++)
+            for (int x = 0; x < sw; x++)
+                for (int ch = 0; ch < 4; ch++) {
+                    int32_t ss = 1 << (kResampleBits - 1);
+                    for (int t = 0; t < c.n[y]; t++)
+                        ss += src[((size_t)(c.lo[y] + t) * sw + x) * 4 + ch] * c.k[(size_t)y * c.ksize + t];
+                    out[((size_t)y * sw + x) * 4 + ch] = clip8(ss);
+                }
+    } else {
+        std::memcpy(out, src, (size_t)sw * h * 4);
+    }
+    return 0;
+}

@@ -31,7 +31,7 @@ HIP_FLAGS = [
     '-Wall', '-Wno-unused-function', '-Wno-unknown-pragmas',
     '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC,
 ]
-HIP_SOURCES = ['kernels.hip', 'capi.cpp', 'qtx_tables.cpp', 'batch.cpp', 'server.cpp']
+HIP_SOURCES = ['kernels.hip', 'resample.hip', 'capi.cpp', 'qtx_tables.cpp', 'batch.cpp', 'server.cpp', 'mips.cpp']
 
 
 def _run(cmd, verbose):

@@ -149,6 +149,14 @@ class Qtx:
         L.qtx_server_now.argtypes = [vp]
         L.qtx_server_now.restype = ctypes.c_double
         L.qtx_server_stop.argtypes = [vp]
+        PP = ctypes.POINTER(ctypes.c_void_p)
+        L.qtx_mip_count.argtypes = [i32, i32, i32]
+        L.qtx_mip_count.restype = i32
+        L.qtx_mips_workspace_bytes.argtypes = [i32, i32, i32]
+        L.qtx_mips_workspace_bytes.restype = sz
+        L.qtx_mips_device.argtypes = [vp, vp, i32, i32, i32, PP, vp, sz, vp]
+        L.qtx_mips_host.argtypes = [vp, _U8P, i32, i32, i32, PP, ctypes.POINTER(sz)]
+        L.qtx_mips_encode_host.argtypes = [vp, JP, _U8P, i32, i32, i32, PP, ctypes.POINTER(sz)]
         self._ctx = None
 
     # -- plumbing -------------------------------------------------------------
@@ -343,6 +351,39 @@ class Server:
             self.q.lib.qtx_server_stop(self.h)
             self.h = None
             self._keep.clear()
+
+
+def mip_sizes(w, h, levels):
+    """Sizes of the first `levels` levels of the chain (level 0 = (w, h))."""
+    out = [(w, h)]
+    for _ in range(levels - 1):
+        w, h = max(w // 2, 1), max(h // 2, 1)
+        out.append((w, h))
+    return out
+
+
+def mips_host(q, rgba, levels):
+    """qtx_mips_host: levels 1.. of the chain of an (h, w, 4) uint8 array, as arrays."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    outs = [np.zeros((lh, lw, 4), np.uint8) for lw, lh in mip_sizes(w, h, levels)[1:]]
+    n = max(len(outs), 1)
+    ptrs = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    caps = (ctypes.c_size_t * n)(*[o.nbytes for o in outs])
+    q.check(q.lib.qtx_mips_host(q.ctx, _ptr(rgba), w, h, levels, ptrs, caps))
+    return outs
+
+
+def mips_encode_host(q, tmpl, rgba, levels):
+    """qtx_mips_encode_host: the encoded levels 0.. of the chain, as uint8 arrays."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    bpb = 8 if tmpl.format in (FMT_BC1, FMT_BC4) else 16
+    outs = [np.zeros(_nb(lw, lh) * bpb, np.uint8) for lw, lh in mip_sizes(w, h, levels)]
+    ptrs = (ctypes.c_void_p * levels)(*[o.ctypes.data for o in outs])
+    caps = (ctypes.c_size_t * levels)(*[o.nbytes for o in outs])
+    q.check(q.lib.qtx_mips_encode_host(q.ctx, ctypes.byref(tmpl), _ptr(rgba), w, h, levels, ptrs, caps))
+    return outs
 
 
 def run_host_batch(q, jobs):

@@ -14,6 +14,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdlib>
 #include <cstring>
@@ -506,6 +507,135 @@ template <class B> void bind_block_texture(py::module_ &m, const char *name) {
     }, "key"_a, "value"_a);
 }
 
+// ---------------------------------------------------------------------------
+// Mip chains and whole-file decodes: the per-level loops of quicktex.dds
+// (encode, quicktex/dds.py:282-287; DDSFile.decode_all, dds.py:209-216) as one
+// call each.
+
+template <class B>
+py::list encode_mips_as(const qtx_job &tmpl, const py::buffer &data, int w, int h, int levels) {
+    if (w <= 0) throw std::invalid_argument("Texture width must be greater than 0");
+    if (h <= 0) throw std::invalid_argument("Texture height must be greater than 0");
+    py::buffer_info info = request_bytes(data, (size_t)w * (size_t)h * 4);
+    std::vector<std::unique_ptr<BlockTexture<B>>> outs;
+    std::vector<uint8_t *> ptrs;
+    std::vector<size_t> caps;
+    int lw = w, lh = h;
+    for (int m = 0; m < levels; m++) {
+        outs.push_back(std::make_unique<BlockTexture<B>>(lw, lh));
+        ptrs.push_back(outs.back()->data());
+        caps.push_back(outs.back()->nbytes());
+        lw = std::max(lw / 2, 1);
+        lh = std::max(lh / 2, 1);
+    }
+    qtx_status st;
+    {
+        py::gil_scoped_release nogil;
+        st = qtx_mips_encode_host(context(), &tmpl, static_cast<const uint8_t *>(info.ptr), w, h, levels, ptrs.data(),
+                                  caps.data());
+    }
+    check(st);
+    py::list l;
+    for (auto &o : outs) l.append(py::cast(std::move(o)));
+    return l;
+}
+
+qtx_job encode_template(int format) {
+    qtx_job j{};
+    j.op = QTX_OP_ENCODE;
+    j.format = format;
+    return j;
+}
+
+// encoder: a BC1/BC3/BC4/BC5 encoder; data: RGBA8 bytes of the base level.
+py::list encode_mips(py::object encoder, py::buffer data, int w, int h, int levels) {
+    if (py::isinstance<BC1Encoder>(encoder)) {
+        qtx_job j = encode_template(QTX_FMT_BC1);
+        j.bc1 = encoder.cast<const BC1Encoder &>().p;
+        return encode_mips_as<BC1Block>(j, data, w, h, levels);
+    }
+    if (py::isinstance<BC3Encoder>(encoder)) {
+        qtx_job j = encode_template(QTX_FMT_BC3);
+        j.bc1 = encoder.cast<const BC3Encoder &>().bc1->p;
+        return encode_mips_as<BC3Block>(j, data, w, h, levels);
+    }
+    if (py::isinstance<BC4Encoder>(encoder)) {
+        qtx_job j = encode_template(QTX_FMT_BC4);
+        j.chan0 = encoder.cast<const BC4Encoder &>().channel;
+        return encode_mips_as<BC4Block>(j, data, w, h, levels);
+    }
+    if (py::isinstance<BC5Encoder>(encoder)) {
+        const BC5Encoder &e = encoder.cast<const BC5Encoder &>();
+        qtx_job j = encode_template(QTX_FMT_BC5);
+        j.chan0 = e.e0->channel;
+        j.chan1 = e.e1->channel;
+        return encode_mips_as<BC5Block>(j, data, w, h, levels);
+    }
+    throw py::type_error("expected a BC1, BC3, BC4 or BC5 encoder");
+}
+
+template <class B> py::list decode_all_as(qtx_job tmpl, const py::list &textures) {
+    const size_t n = py::len(textures);
+    std::vector<std::unique_ptr<RawTexture>> outs;
+    std::vector<qtx_job> jobs(n, tmpl);
+    std::vector<size_t> caps(n);
+    for (size_t i = 0; i < n; i++) {
+        const BlockTexture<B> &t = textures[i].cast<const BlockTexture<B> &>();
+        outs.push_back(std::make_unique<RawTexture>(t.width, t.height));
+        jobs[i].width = t.width;
+        jobs[i].height = t.height;
+        jobs[i].src = t.cdata();
+        jobs[i].dst = outs.back()->data();
+        caps[i] = outs.back()->nbytes();
+    }
+    py::list l;
+    if (n == 0) return l;
+    qtx_status st;
+    {
+        py::gil_scoped_release nogil;
+        st = qtx_batch_run_host(context(), jobs.data(), caps.data(), n);
+    }
+    check(st);
+    for (auto &o : outs) l.append(py::cast(std::move(o)));
+    return l;
+}
+
+qtx_job decode_template(int format) {
+    qtx_job j{};
+    j.op = QTX_OP_DECODE;
+    j.format = format;
+    return j;
+}
+
+// decoder: a BC1/BC3/BC4/BC5 decoder; textures: block textures of its format.
+py::list decode_all(py::object decoder, py::list textures) {
+    if (py::isinstance<BC1Decoder>(decoder)) {
+        const BC1Decoder &d = decoder.cast<const BC1Decoder &>();
+        qtx_job j = decode_template(QTX_FMT_BC1);
+        j.bc1.interpolator = d.interpolator->type();
+        j.write_alpha = d.write_alpha;
+        return decode_all_as<BC1Block>(j, textures);
+    }
+    if (py::isinstance<BC3Decoder>(decoder)) {
+        qtx_job j = decode_template(QTX_FMT_BC3);
+        j.bc1.interpolator = decoder.cast<const BC3Decoder &>().bc1->interpolator->type();
+        return decode_all_as<BC3Block>(j, textures);
+    }
+    if (py::isinstance<BC4Decoder>(decoder)) {
+        qtx_job j = decode_template(QTX_FMT_BC4);
+        j.chan0 = decoder.cast<const BC4Decoder &>().channel;
+        return decode_all_as<BC4Block>(j, textures);
+    }
+    if (py::isinstance<BC5Decoder>(decoder)) {
+        const BC5Decoder &d = decoder.cast<const BC5Decoder &>();
+        qtx_job j = decode_template(QTX_FMT_BC5);
+        j.chan0 = d.d0->channel;
+        j.chan1 = d.d1->channel;
+        return decode_all_as<BC5Block>(j, textures);
+    }
+    throw py::type_error("expected a BC1, BC3, BC4 or BC5 decoder");
+}
+
 }  // namespace qb
 
 PYBIND11_MODULE(_quicktex, m) {
@@ -660,4 +790,10 @@ PYBIND11_MODULE(_quicktex, m) {
     d5.def("decode", &BC5Decoder::decode, "texture"_a);
     d5.def_property_readonly("channels", [](const BC5Decoder &d) { return std::make_tuple(d.d0->channel, d.d1->channel); });
     d5.def_property_readonly("bc4_decoders", [](const BC5Decoder &d) { return std::make_tuple(d.d0, d.d1); });
+
+    // whole mip chains (quicktex.dds)
+    m.def("_mip_count", [](int w, int h, int requested) { return qtx_mip_count(w, h, requested); }, "width"_a,
+          "height"_a, "requested"_a = 0);
+    m.def("_encode_mips", &encode_mips, "encoder"_a, "data"_a, "width"_a, "height"_a, "levels"_a);
+    m.def("_decode_all", &decode_all, "decoder"_a, "textures"_a);
 }

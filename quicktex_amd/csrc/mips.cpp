@@ -1,0 +1,156 @@
+// mips.cpp -- the C ABI of the mip chain (include/qtx.h, qtx_mip*): Pillow-exact
+// levels resampled on the GPU (resample.h) and, for the DDS encoder, fed to one
+// batched encode without leaving the device.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "batch_internal.h"
+#include "resample.h"
+#include "runtime.h"
+
+using namespace qtx::rt;
+
+namespace {
+
+// Validates a chain request; fills the plan and its level count.
+qtx_status chain_plan(int32_t w, int32_t h, int32_t levels, qtx::MipPlan &p, size_t &ws_bytes) {
+    qtx_status st = check_dims(w, h);
+    if (st != QTX_OK) return st;
+    if (levels < 1) return fail(QTX_EINVAL, "a mip chain has at least one level (got %d)", levels);
+    const int full = qtx::mip_level_count(w, h, 0);
+    if (levels > full) return fail(QTX_EINVAL, "a %d x %d texture has %d mip levels, not %d", w, h, full, levels);
+    ws_bytes = qtx::mip_plan(w, h, levels, p);
+    return QTX_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t qtx_mip_count(int32_t width, int32_t height, int32_t requested) {
+    if (width <= 0 || height <= 0) return 0;
+    return qtx::mip_level_count(width, height, requested);
+}
+
+size_t qtx_mips_workspace_bytes(int32_t width, int32_t height, int32_t levels) {
+    qtx::MipPlan p;
+    size_t ws = 0;
+    if (chain_plan(width, height, levels, p, ws) != QTX_OK) return 0;
+    return ws;
+}
+
+qtx_status qtx_mips_device(qtx_ctx *ctx, const uint8_t *d_rgba, int32_t width, int32_t height, int32_t levels,
+                           uint8_t *const *d_levels, void *d_workspace, size_t workspace_bytes, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    qtx::MipPlan p;
+    size_t need = 0;
+    if ((st = chain_plan(width, height, levels, p, need)) != QTX_OK) return st;
+    if (!d_rgba || (levels > 1 && (!d_levels || !d_workspace))) return fail(QTX_EINVAL, "null buffer");
+    if (workspace_bytes < need) return fail(QTX_EINVAL, "workspace holds %zu bytes, %zu needed", workspace_bytes, need);
+    for (int m = 1; m < levels; m++) {
+        if (!d_levels[m - 1]) return fail(QTX_EINVAL, "null buffer for level %d", m);
+        p.dst[m] = d_levels[m - 1];
+    }
+    QTX_HIP(qtx::launch_mips(p, d_rgba, static_cast<uint8_t *>(d_workspace), (hipStream_t)stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_mips_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, int32_t height, int32_t levels,
+                         uint8_t *const *out_levels, const size_t *out_bytes) {
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    qtx::MipPlan p;
+    size_t ws = 0;
+    if ((st = chain_plan(width, height, levels, p, ws)) != QTX_OK) return st;
+    if (!rgba || (levels > 1 && (!out_levels || !out_bytes))) return fail(QTX_EINVAL, "null buffer");
+    std::vector<size_t> off(levels);
+    size_t total = align256(raw_bytes(width, height));
+    for (int m = 1; m < levels; m++) {
+        const size_t n = raw_bytes(p.lw[m], p.lh[m]);
+        if (!out_levels[m - 1]) return fail(QTX_EINVAL, "null buffer for level %d", m);
+        if (out_bytes[m - 1] < n)
+            return fail(QTX_EINVAL, "level %d: output buffer holds %zu bytes, %zu needed", m, out_bytes[m - 1], n);
+        off[m] = total;
+        total += align256(n);
+    }
+    const size_t ws_off = total;
+    total += ws;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
+    QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+    for (int m = 1; m < levels; m++) p.dst[m] = ctx->d_in + off[m];
+    QTX_HIP(qtx::launch_mips(p, ctx->d_in, ctx->d_in + ws_off, ctx->stream));
+    for (int m = 1; m < levels; m++)
+        QTX_HIP(hipMemcpyAsync(out_levels[m - 1], ctx->d_in + off[m], raw_bytes(p.lw[m], p.lh[m]),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    QTX_HIP(hipStreamSynchronize(ctx->stream));
+    return QTX_OK;
+}
+
+qtx_status qtx_mips_encode_host(qtx_ctx *ctx, const qtx_job *tmpl, const uint8_t *rgba, int32_t width, int32_t height,
+                                int32_t levels, uint8_t *const *out_blocks, const size_t *out_bytes) {
+    qtx_status st = check_ctx(ctx);
+    if (st != QTX_OK) return st;
+    if (!tmpl) return fail(QTX_EINVAL, "null job template");
+    if (tmpl->op != QTX_OP_ENCODE) return fail(QTX_EINVAL, "the job template must encode");
+    qtx::MipPlan p;
+    size_t ws = 0;
+    if ((st = chain_plan(width, height, levels, p, ws)) != QTX_OK) return st;
+    if (!rgba || !out_blocks || !out_bytes) return fail(QTX_EINVAL, "null buffer");
+    // layout: base | levels 1.. | workspace | encoded levels
+    std::vector<size_t> img_off(levels), blk_off(levels);
+    std::vector<qtx_job> jobs(levels, *tmpl);
+    std::vector<qtx::bt::Prepared> prep(levels);
+    size_t total = 0;
+    for (int m = 0; m < levels; m++) {
+        img_off[m] = total;
+        total += align256(raw_bytes(p.lw[m], p.lh[m]));
+    }
+    const size_t ws_off = total;
+    total += align256(ws);
+    for (int m = 0; m < levels; m++) {
+        jobs[m].width = p.lw[m];
+        jobs[m].height = p.lh[m];
+        jobs[m].src = rgba;  // validated below; replaced by device pointers
+        jobs[m].dst = out_blocks[m];
+        if ((st = qtx::bt::prepare(jobs[m], (size_t)m, prep[m])) != QTX_OK) return st;
+        if (!out_blocks[m]) return fail(QTX_EINVAL, "null buffer for level %d", m);
+        if (out_bytes[m] < prep[m].dst_bytes)
+            return fail(QTX_EINVAL, "level %d: output buffer holds %zu bytes, %zu needed", m, out_bytes[m],
+                        prep[m].dst_bytes);
+        blk_off[m] = total;
+        total += align256(prep[m].dst_bytes);
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
+    uint8_t *d = ctx->d_in;
+    QTX_HIP(hipMemcpyAsync(d, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+    for (int m = 1; m < levels; m++) p.dst[m] = d + img_off[m];
+    QTX_HIP(qtx::launch_mips(p, d, d + ws_off, ctx->stream));
+    for (int m = 0; m < levels; m++) {
+        jobs[m].src = d + img_off[m];
+        jobs[m].dst = d + blk_off[m];
+    }
+    qtx_batch *b = nullptr;
+    if ((st = qtx_batch_create(ctx, jobs.data(), (size_t)levels, &b)) != QTX_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st;
+    }
+    st = qtx_batch_launch(b, ctx->stream);
+    for (int m = 0; m < levels && st == QTX_OK; m++) {
+        const hipError_t e = hipMemcpyAsync(out_blocks[m], d + blk_off[m], prep[m].dst_bytes, hipMemcpyDeviceToHost,
+                                            ctx->stream);
+        if (e != hipSuccess) st = hip_fail(e, "hipMemcpyAsync");
+    }
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+    qtx_batch_destroy(b);
+    return st;
+}
+
+}  // extern "C"

@@ -100,6 +100,14 @@ class _Codec:
         self._call('bc5_decode', ctypes.c_uint32(chan0), ctypes.c_uint32(chan1), _ptr(blocks), w, h, _ptr(out))
         return out
 
+    def resize(self, rgba, ow, oh):
+        """Pillow's BILINEAR resize of every RGBA8 channel (the oracle only)."""
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        h, w = rgba.shape[:2]
+        out = np.zeros((oh, ow, 4), np.uint8)
+        self._call('resize', _ptr(rgba), w, h, _ptr(out), ow, oh)
+        return out
+
 
 _oracle = None
 _ref = None

@@ -240,3 +240,56 @@ def test_order_factors_invert_normal_matrix(oracle):
                 continue
             prod = fm.reshape(2, 2) @ m
             assert np.allclose(prod, np.eye(2) * d / 255, atol=1e-6)
+
+
+# -- mip levels: Pillow's BILINEAR resize (quicktex/image_utils.py:42-52) -----------------
+
+def _pil_resize_no_premultiply(rgba, ow, oh):
+    """quicktex.image_utils.resize_no_premultiply on an RGBA image, through Pillow itself."""
+    from PIL import Image
+    im = Image.frombytes('RGBA', (rgba.shape[1], rgba.shape[0]), np.ascontiguousarray(rgba).tobytes())
+    rgb = im.convert('RGB').resize((ow, oh), Image.BILINEAR)
+    rgb.putalpha(im.getchannel('A').resize((ow, oh), Image.BILINEAR))
+    return np.asarray(rgb)
+
+
+def _chain(w, h):
+    out = []
+    while (w, h) != (1, 1):
+        w, h = max(w // 2, 1), max(h // 2, 1)
+        out.append((w, h))
+    return out
+
+
+@pytest.mark.parametrize('w,h', [(1, 2), (2, 1), (3, 3), (5, 7), (13, 1), (1, 29), (37, 23), (64, 64),
+                                 (255, 3), (256, 256), (1000, 600), (513, 257)])
+def test_resize_matches_pillow(oracle, w, h):
+    pytest.importorskip('PIL')
+    rng = np.random.default_rng(w * 7919 + h)
+    img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    for ow, oh in _chain(w, h) + [(max(1, w * 2 // 3), max(1, h * 3 // 5)), (w + 3, h + 1)]:
+        got = oracle.resize(img, ow, oh)
+        want = _pil_resize_no_premultiply(img, ow, oh)
+        assert (got == want).all(), (w, h, ow, oh)
+
+
+def test_resize_rgbx_equals_per_channel(oracle):
+    """Image.resize on RGBX (dds.encode of an image without alpha) resamples the
+    fourth band like the others: the same as the per-channel restatement."""
+    from PIL import Image
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (45, 70, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    im = Image.frombytes('RGBX', (70, 45), img.tobytes())
+    for ow, oh in _chain(70, 45):
+        want = np.frombuffer(im.resize((ow, oh), Image.BILINEAR).tobytes('raw', 'RGBX'), np.uint8).reshape(oh, ow, 4)
+        assert (oracle.resize(img, ow, oh) == want).all(), (ow, oh)
+
+
+def test_resize_closed_forms(oracle):
+    """Constant images stay constant; a 2:1 shrink of a 2-pixel row is the rounded mean."""
+    img = np.full((9, 11, 4), 77, np.uint8)
+    assert (oracle.resize(img, 5, 4) == 77).all()
+    row = np.array([[[10, 20, 30, 40], [11, 25, 32, 200]]], np.uint8)
+    want = (row.astype(int).sum(1) + 1) // 2  # weights 1/2, 1/2 -> (a + b + 1) >> 1
+    assert (oracle.resize(row, 1, 1)[0, 0] == want[0]).all()
