@@ -48,10 +48,10 @@ METRIC = 'BC1 L18 encode Mpixel/s bit-exact, batched/served vs plain; p50/p99 la
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # non-packed 32-bit lane-ops per second, T
 W = H = 4096
 INTERPS = ('Ideal', 'IdealRound', 'Nvidia', 'AMD')
-# HBM bytes per BC1 encode launch on this workload (k_bc1_lane), from the committed
-# counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2.000 + WRITE_SIZE x
-# 1.000 (copy-kernel calibration), mean over the four interpolators.
-TRAFFIC_BYTES = (141.4e6 + 141.5e6 + 140.7e6 + 161.2e6 + 166.7e6 + 166.6e6 + 166.1e6 + 184.6e6) / 4
+# HBM bytes per BC1 encode launch on this workload (k_bc1_lane_queue), from the
+# committed counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 (copy-kernel calibration), mean over the four interpolators.
+TRAFFIC_BYTES = (286.39e6 + 286.83e6 + 284.13e6 + 286.90e6) / 4
 
 
 def log(*a):
@@ -121,7 +121,7 @@ def plain_leg(a, q, hip, np):
     achieved = [ops_launch[i] / (dur[:, i].mean() / 1000.0) / 1e12 for i in range(4)]
     achieved_mean = float(np.mean(achieved))
     roofline = {
-        'kernel': 'k_bc1_lane (BC1 level 18, ThreeColorBlack, one block per lane)',
+        'kernel': 'k_bc1_lane_queue (BC1 level 18, ThreeColorBlack, one block per lane, queued dealing)',
         'bound': 'valu', 'unit': 'TOP/s',
         'achieved': round(achieved_mean, 3), 'peak': round(VALU_PEAK_TOPS, 2),
         'frac': round(achieved_mean / VALU_PEAK_TOPS, 4),
