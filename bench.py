@@ -33,6 +33,12 @@ plain host entry point, one after another; (b) all submitted at once to the
 serving loop; (c) submitted at Poisson arrivals of a fixed rate, for p50/p99
 latency.  Outputs of a sample of requests must be byte-equal across (a)-(c).
 
+dds: quicktex.dds.encode's path -- a 4096x4096 RGBA8 image and its full mip
+chain, every level resampled from the base with Pillow's BILINEAR filter, BC3
+level 10: the resample kernels alone (HBM GB/s), resample + batched encode
+device-resident, and host to host; the reference's flow (Pillow resize + its
+BC3 encoder per level) timed on a 1024x1024 crop and checked bit-exact.
+
 Prints one JSON line.
 """
 import argparse
@@ -509,6 +515,120 @@ def sweep_leg(a, q, hip, np):
 
 
 # ---------------------------------------------------------------------------
+def dds_leg(a, q, hip, np):
+    """The DDS encoder's path (quicktex.dds.encode): a mip chain resampled from the
+    base with Pillow's BILINEAR filter, every level BC3 level-10 encoded."""
+    import ctypes
+
+    from quicktex_amd import synth
+    from quicktex_amd.capi import COLOR_FOUR, FMT_BC3, OP_ENCODE, Batch, job, mip_sizes, mips_encode_host
+
+    S = a.dds_size
+    img = synth.mixture_rgba(S, S, seed=0x5EED0005)
+    img[..., 3] = synth.alpha_channel(S, S, seed=0x5EED0006)
+    levels = q.lib.qtx_mip_count(S, S, 0)
+    sizes = mip_sizes(S, S, levels)
+    npx = sum(w * h for w, h in sizes)
+    params = q.params(10, COLOR_FOUR, 0)
+    stream = hip.Stream()
+    sh = stream.handle
+
+    # (1) the resample kernels alone, device-resident
+    d_img = hip.Buffer.from_array(img)
+    d_levels = [hip.Buffer(w * h * 4) for w, h in sizes[1:]]
+    ws_bytes = q.lib.qtx_mips_workspace_bytes(S, S, levels)
+    d_ws = hip.Buffer(ws_bytes)
+    ptrs = (ctypes.c_void_p * (levels - 1))(*[b.ptr for b in d_levels])
+
+    def mips():
+        q.check(q.lib.qtx_mips_device(q.ctx, d_img.ptr, S, S, levels, ptrs, d_ws.ptr, ws_bytes, sh))
+
+    def events(fn, reps):
+        fn()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    mips_ms = events(mips, 10)
+    # each level reads the base once and writes itself
+    alg_bytes = sum(S * S * 4 + w * h * 4 for w, h in sizes[1:])
+    log('dds: mip resample %.3f ms for %d levels, %.0f GB/s algorithmic' % (mips_ms, levels - 1,
+                                                                         alg_bytes / mips_ms / 1e6))
+
+    # (2) device-resident chain encode: resample + one batch over the levels
+    d_blocks = [hip.Buffer(((w + 3) // 4) * ((h + 3) // 4) * 16) for w, h in sizes]
+    srcs = [d_img.ptr] + [b.ptr for b in d_levels]
+    b = Batch(q, [job(OP_ENCODE, FMT_BC3, w, h, s, d.ptr, params=params)
+                  for (w, h), s, d in zip(sizes, srcs, d_blocks)])
+
+    def chain():
+        mips()
+        b.launch(sh)
+
+    chain_ms = events(chain, 5)
+    log('dds: device chain (resample + batch encode) %.3f ms, %.1f Mpx/s' % (chain_ms, npx / chain_ms / 1e3))
+
+    # (3) host to host through qtx_mips_encode_host, as quicktex_amd.dds.encode calls it
+    t = job(OP_ENCODE, FMT_BC3, 0, 0, 0, 0, params=params)
+    mips_encode_host(q, t, img, levels)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        host = mips_encode_host(q, t, img, levels)
+    host_ms = (time.perf_counter() - t0) * 1000.0 / reps
+    log('dds: host to host %.2f ms, %.1f Mpx/s' % (host_ms, npx / host_ms / 1e3))
+    same = all(np.array_equal(h, np.frombuffer(d.download(), np.uint8)[:h.size]) for h, d in zip(host, d_blocks))
+    b.close()
+
+    cpu = None
+    if not a.no_cpu:
+        # the reference's flow on a crop: Pillow resize of each level from the base,
+        # then the reference's BC3 encoder per level (one thread, Encoder.h:46,58)
+        from PIL import Image
+        lib, kind = cpu_lib()
+        C = a.dds_cpu_size
+        crop = np.ascontiguousarray(img[:C, :C])
+        im = Image.frombytes('RGBA', (C, C), crop.tobytes())
+        csizes = mip_sizes(C, C, q.lib.qtx_mip_count(C, C, 0))
+        lib.bc3_encode(np.ascontiguousarray(crop[:4, :4]), params)  # construction, not timed
+        t0 = time.perf_counter()
+        outs = []
+        for m, (w, h) in enumerate(csizes):
+            if m == 0:
+                lvl = crop
+            else:
+                rgb = im.convert('RGB').resize((w, h), Image.BILINEAR)
+                rgb.putalpha(im.getchannel('A').resize((w, h), Image.BILINEAR))
+                lvl = np.asarray(rgb)
+            outs.append(lib.bc3_encode(lvl, params))
+        secs = time.perf_counter() - t0
+        cpx = sum(w * h for w, h in csizes)
+        gpu = mips_encode_host(q, t, crop, len(csizes))
+        exact = all(np.array_equal(g, o.reshape(-1)) for g, o in zip(gpu, outs))
+        cpu = {'kind': kind, 'value': round(cpx / secs / 1e6, 3), 'unit': 'Mpixel/s', 'cores': 1,
+               'sample': '%dx%d crop of the same image, full chain (%d levels), Pillow resize + BC3 level 10 '
+                         'per level' % (C, C, len(csizes)),
+               'bit_exact_vs_gpu': exact}
+        log('dds cpu %s: %.3f Mpx/s, exact %s' % (kind, cpu['value'], exact))
+    stream.destroy()
+    return {
+        'workload': 'DDS encode: %dx%d RGBA8 + full mip chain (%d levels, Pillow BILINEAR from the base), '
+                    'BC3 level 10, %.1f Mpixel' % (S, S, levels, npx / 1e6),
+        'mips_ms': round(mips_ms, 4),
+        'mips_algorithmic_gb_s': round(alg_bytes / mips_ms / 1e6, 1),
+        'mips_hbm_frac': round(alg_bytes / mips_ms / 1e6 / 8000.0, 4),
+        'device_chain_ms': round(chain_ms, 3), 'device_chain_mpx_s': round(npx / chain_ms / 1e3, 1),
+        'host_to_host_ms': round(host_ms, 2), 'host_to_host_mpx_s': round(npx / host_ms / 1e3, 1),
+        'host_equals_device': bool(same),
+        'cpu': cpu,
+    }
+
+
+# ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -516,7 +636,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--legs', default='plain,batched,served', help='comma list of: plain, batched, served, sweep')
+    ap.add_argument('--legs', default='plain,batched,served,dds',
+                    help='comma list of: plain, batched, served, dds, sweep')
     ap.add_argument('--serve-requests', type=int, default=2000)
     ap.add_argument('--serve-rate', type=float, default=0.0,
                     help='Poisson arrival rate, requests/s (default: 60%% of the measured saturation rate)')
@@ -524,6 +645,8 @@ def main():
     ap.add_argument('--sweep-size', type=int, default=4096, help='texture side of the sweep leg')
     ap.add_argument('--batch-textures', type=int, default=300)
     ap.add_argument('--batch-steps', type=int, default=5)
+    ap.add_argument('--dds-size', type=int, default=4096, help='base side of the dds leg')
+    ap.add_argument('--dds-cpu-size', type=int, default=1024, help='base side of the dds leg\'s CPU sample')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run 3 device-to-device copies of MIB MiB (calibrates the HBM byte counters) and exit')
     a = ap.parse_args()
@@ -567,6 +690,8 @@ def main():
         line['batched'] = batched_leg(a, q, hip, np)
     if 'served' in legs:
         line['served'] = served_leg(a, q, hip, np)
+    if 'dds' in legs:
+        line['dds'] = dds_leg(a, q, hip, np)
     if 'sweep' in legs:
         line['sweep'] = sweep_leg(a, q, hip, np)
     print(json.dumps(line), flush=True)

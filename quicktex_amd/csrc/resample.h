@@ -29,6 +29,8 @@ namespace qtx {
 
 constexpr int kMaxMipLevels = 32;  // a 2^31-texel texture has at most 32 levels
 constexpr int kResampleWideTaps = 64;  // windows longer than this use a wave per output
+constexpr int kRowMaxWidth = 8192;     // widest base whose rows the horizontal pass keeps in LDS
+constexpr int kRowGrid = 2048;         // workgroups of that pass (they stride over the rows)
 
 // Layout of one mip chain in a caller's workspace, and its launch ranges.  Passed
 // to the kernels by value.
@@ -39,10 +41,12 @@ struct MipPlan {
     uint64_t cx[kMaxMipLevels], cy[kMaxMipLevels];   // workspace offsets (int32 words) of the
     uint64_t wx[kMaxMipLevels], wy[kMaxMipLevels];   //   coefficient and window (lo, n) tables
     uint64_t tmp[kMaxMipLevels];                     // byte offset of the horizontal-pass image
-    // first thread / unit of each level in the coefficient, horizontal and vertical
+    // first wave / unit of each level in the coefficient, horizontal and vertical
     // launches (prefix sums; entry `levels` is the total)
     uint32_t coef0[kMaxMipLevels + 1];
     uint32_t h0[kMaxMipLevels + 1], v0[kMaxMipLevels + 1];
+    int32_t gx[kMaxMipLevels];        // lanes per output in the horizontal pass over LDS rows
+    uint32_t exact_x, exact_y;        // bit m: that axis of level m shrinks by a power of two
     uint8_t *dst[kMaxMipLevels];  // level outputs, RGBA8 (dst[0] unused)
 };
 

@@ -15,33 +15,77 @@ constexpr int kBits = 22;  // fraction bits of the fixed-point weights
 
 // Window and weights of output index i of an axis of in_size -> out_size
 // (libImaging/Resample.c, precompute_coeffs + normalize_coeffs_8bpc, bilinear
-// filter): the same double operations in the same order as Pillow.
-__device__ void axis_coeffs(int in_size, int out_size, int i, int32_t *__restrict__ k, int32_t *__restrict__ win) {
+// filter): the same double operations in the same order as Pillow.  The sum of
+// the weights is sequential, in tap order, as Pillow's (double addition is not
+// associative); everything else is per tap, so a wave shares a long window:
+// `lanes` lanes with ids `lane` (one lane for a short window, 64 for a long one).
+struct Window {
+    double center, ss;
+    int xmin, n;
+    __device__ double weight(int x) const {
+        double t = (x + xmin - center + 0.5) * ss;
+        if (t < 0.0) t = -t;
+        return t < 1.0 ? 1.0 - t : 0.0;
+    }
+};
+
+__device__ Window axis_window(int in_size, int out_size, int i) {
     const double scale = (double)in_size / (double)out_size;
     const double filterscale = scale < 1.0 ? 1.0 : scale;
     const double support = 1.0 * filterscale;
-    const double center = 0.0 + (i + 0.5) * scale;
-    const double ss = 1.0 / filterscale;
-    int xmin = (int)(center - support + 0.5);
+    Window wd;
+    wd.center = 0.0 + (i + 0.5) * scale;
+    wd.ss = 1.0 / filterscale;
+    int xmin = (int)(wd.center - support + 0.5);
     if (xmin < 0) xmin = 0;
-    int xmax = (int)(center + support + 0.5);
+    int xmax = (int)(wd.center + support + 0.5);
     if (xmax > in_size) xmax = in_size;
-    xmax -= xmin;
+    wd.xmin = xmin;
+    wd.n = xmax - xmin;
+    return wd;
+}
+
+// The double of lane j (j uniform across the wave).
+__device__ __forceinline__ double lane_double(double v, int j) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+
+// Weights of output index i, tap x stored at k[x * stride] (tables are tap-major,
+// so that the lanes of a pass, on consecutive outputs, read consecutive words).
+// lanes == 1: one lane does everything.  lanes == 64: the wave computes 64 taps
+// at a time and every lane adds them to the sum in tap order (each value read
+// from its lane), so the sum is Pillow's sequential one.
+__device__ void axis_coeffs(int in_size, int out_size, int i, int lane, int lanes, bool exact,
+                            int32_t *__restrict__ k, int stride, int32_t *__restrict__ win) {
+    const Window wd = axis_window(in_size, out_size, i);
     double ww = 0.0;
-    for (int x = 0; x < xmax; x++) {
-        double t = (x + xmin - center + 0.5) * ss;
-        if (t < 0.0) t = -t;
-        ww += t < 1.0 ? 1.0 - t : 0.0;
+    if (lanes == 1) {
+        for (int x = 0; x < wd.n; x++) ww += wd.weight(x);
+    } else if (exact) {
+        // a power-of-two scale: every weight is a dyadic fraction with at most
+        // log2(scale) + 1 fraction bits and every partial sum is exact, so the
+        // sum does not depend on the order of the additions
+        for (int x = lane; x < wd.n; x += 64) ww += wd.weight(x);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) ww += __shfl_xor(ww, d, 64);
+    } else {
+        for (int b = 0; b < wd.n; b += 64) {
+            const double wx = b + lane < wd.n ? wd.weight(b + lane) : 0.0;
+            const int cnt = min(64, wd.n - b);
+            for (int j = 0; j < cnt; j++) ww += lane_double(wx, j);
+        }
     }
-    for (int x = 0; x < xmax; x++) {
-        double t = (x + xmin - center + 0.5) * ss;
-        if (t < 0.0) t = -t;
-        double v = t < 1.0 ? 1.0 - t : 0.0;
+    for (int x = lane; x < wd.n; x += lanes) {
+        double v = wd.weight(x);
         if (ww != 0.0) v /= ww;
-        k[x] = v < 0.0 ? (int32_t)(-0.5 + v * (1 << kBits)) : (int32_t)(0.5 + v * (1 << kBits));
+        k[(size_t)x * stride] = v < 0.0 ? (int32_t)(-0.5 + v * (1 << kBits)) : (int32_t)(0.5 + v * (1 << kBits));
     }
-    win[0] = xmin;
-    win[1] = xmax;
+    if (lane == 0) {
+        win[0] = wd.xmin;
+        win[1] = wd.n;
+    }
 }
 
 // Level of a launch-wide index: the last level whose first index is <= u.
@@ -52,20 +96,44 @@ __device__ __forceinline__ int level_of(const uint32_t *first, int levels, uint3
     return m;
 }
 
-// One thread per output index of every level and axis: x indices, then y.
-__global__ __launch_bounds__(256) void k_mip_coeffs(MipPlan p, int32_t *__restrict__ ws) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= p.coef0[p.levels]) return;
-    const int m = level_of(p.coef0, p.levels, t);
-    const int i = (int)(t - p.coef0[m]);
-    if (i < p.lw[m]) axis_coeffs(p.w, p.lw[m], i, ws + p.cx[m] + (size_t)i * p.ksx[m], ws + p.wx[m] + 2 * i);
-    else axis_coeffs(p.h, p.lh[m], i - p.lw[m], ws + p.cy[m] + (size_t)(i - p.lw[m]) * p.ksy[m],
-                     ws + p.wy[m] + 2 * (i - p.lw[m]));
+// Coefficients of every level and axis, in waves: an axis whose window is at most
+// kResampleWideTaps taps long takes 64 output indices per wave (one per lane), a
+// longer one one index per wave.  coef0 counts waves: a level's x waves, then its
+// y waves.
+__device__ __forceinline__ uint32_t axis_waves(int out_size, int ks) {
+    return ks > kResampleWideTaps ? (uint32_t)out_size : (uint32_t)((out_size + 63) / 64);
 }
 
+__global__ __launch_bounds__(256) void k_mip_coeffs(MipPlan p, int32_t *__restrict__ ws) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    if (wave >= p.coef0[p.levels]) return;  // wave-uniform
+    const int m = level_of(p.coef0, p.levels, wave);
+    uint32_t local = wave - p.coef0[m];
+    const uint32_t xw = axis_waves(p.lw[m], p.ksx[m]);
+    const bool yaxis = local >= xw;
+    if (yaxis) local -= xw;
+    const int in_size = yaxis ? p.h : p.w, out_size = yaxis ? p.lh[m] : p.lw[m];
+    const int ks = yaxis ? p.ksy[m] : p.ksx[m];
+    int32_t *kt = ws + (yaxis ? p.cy[m] : p.cx[m]);
+    int32_t *wt = ws + (yaxis ? p.wy[m] : p.wx[m]);
+    if (ks > kResampleWideTaps) {
+        const int i = (int)local;
+        const bool exact = ((yaxis ? p.exact_y : p.exact_x) >> m) & 1u;
+        axis_coeffs(in_size, out_size, i, lane, 64, exact, kt + i, out_size, wt + 2 * i);
+    } else {
+        const int i = (int)local * 64 + lane;
+        if (i < out_size) axis_coeffs(in_size, out_size, i, 0, 1, false, kt + i, out_size, wt + 2 * i);
+    }
+}
+
+// Weights of the triangle filter are never negative and at most 2^22 and texels
+// at most 255, so every product fits the 24-bit multiplier (v_mad_u32_u24, full
+// rate; a 32-bit multiply is a quarter-rate instruction) and every sum, at most
+// 2^21 + 255 * (2^22 + 64), fits 31 bits.
 __device__ __forceinline__ void accumulate(uint32_t px, int32_t k, int32_t (&ss)[4]) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) ss[c] += (int32_t)((px >> (8 * c)) & 0xFFu) * k;
+    for (int c = 0; c < 4; c++) ss[c] += (int32_t)__umul24((px >> (8 * c)) & 0xFFu, (uint32_t)k);
 }
 
 // The four clamped channels packed as RGBA8.  The optimisation barrier keeps the
@@ -82,6 +150,23 @@ __device__ __forceinline__ uint32_t pack_clip(const int32_t (&ss)[4]) {
         asm volatile("" : "+v"(q));
         v |= q << (8 * c);
     }
+    return v;
+}
+
+// Sum over aligned groups of G lanes (G = 1, 2, 4, ..., 64, uniform) with DPP:
+// quad swaps and row mirrors leave each 2/4/8/16-lane group's sum in all its
+// lanes; row broadcasts then add rows 0+1 into row 1 and 2+3 into row 3 (G = 32)
+// and rows 0-1 into rows 2-3 (G = 64).  The last lane of every group holds the sum.
+template <int CTRL, int ROWS = 0xF> __device__ __forceinline__ int32_t dpp_add(int32_t v) {
+    return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ int32_t group_sum(int32_t v, int G) {
+    if (G >= 2) v = dpp_add<0xB1>(v);         // quad_perm [1,0,3,2]
+    if (G >= 4) v = dpp_add<0x4E>(v);         // quad_perm [2,3,0,1]
+    if (G >= 8) v = dpp_add<0x141>(v);        // row_half_mirror
+    if (G >= 16) v = dpp_add<0x140>(v);       // row_mirror
+    if (G >= 32) v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    if (G >= 64) v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -129,18 +214,69 @@ __global__ __launch_bounds__(256) void k_mip_pass(MipPlan p, const uint8_t *__re
     const int x = (int)(o % (uint64_t)lw), y = (int)(o / (uint64_t)lw);
     const int i = VERTICAL ? y : x;  // index along the pass's axis
     const int lo = win[2 * i], n = win[2 * i + 1];
-    const int32_t *k = kt + (size_t)i * ks;
+    const int32_t *k = kt + i;
+    const size_t kstride = VERTICAL ? (size_t)lh : (size_t)lw;  // tap-major tables
     int32_t ss[4] = {0, 0, 0, 0};
     const int t0 = wide ? (int)(threadIdx.x & 63) : 0, dt = wide ? 64 : 1;
     for (int t = t0; t < n; t += dt) {
         const uint32_t px = VERTICAL ? src[(size_t)(lo + t) * sw + x] : src[(size_t)y * sw + lo + t];
-        accumulate(px, k[t], ss);
+        accumulate(px, k[t * kstride], ss);
     }
     if (wide) wave_sum(ss);
     if (!wide || (threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int c = 0; c < 4; c++) ss[c] += 1 << (kBits - 1);
         dst[(size_t)y * lw + x] = pack_clip(ss);
+    }
+}
+
+// The horizontal pass of every level for whole base rows: a workgroup copies a
+// row into LDS once and computes that row of every level from it.  Each output
+// of level m is summed by a group of G = gx[m] lanes (1, or 16 for windows of 32
+// taps and more), lane l of the group taking taps l, l + G, ...; the group adds
+// its partial sums with DPP lane moves (integer sums: any order gives the same
+// total).
+// Rows of at most kRowMaxWidth texels.
+extern __shared__ __align__(16) uint32_t mip_lds[];
+
+__global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__restrict__ base,
+                                                   uint8_t *__restrict__ ws) {
+    uint32_t *row = mip_lds;
+    const int32_t *wsi = reinterpret_cast<const int32_t *>(ws);
+    const int tid = (int)threadIdx.x;
+    for (int y = (int)blockIdx.x; y < p.h; y += (int)gridDim.x) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(base) + (size_t)y * p.w;
+        for (int x = tid; x < p.w; x += 256) row[x] = src[x];
+        __syncthreads();
+        for (int m = 1; m < p.levels; m++) {
+            const int lw = p.lw[m];
+            if (lw == p.w) continue;
+            const int G = p.gx[m], gshift = __builtin_ctz((unsigned)G);
+            const int32_t *kt = wsi + p.cx[m];
+            const int32_t *win = wsi + p.wx[m];
+            uint32_t *dst = reinterpret_cast<uint32_t *>(p.lh[m] != p.h ? ws + p.tmp[m] : p.dst[m]) + (size_t)y * lw;
+            const int lanes = lw * G;
+            for (int b = 0; b < lanes; b += 256) {  // uniform across the workgroup
+                const int it = b + tid;
+                const int o = it >> gshift, l = it & (G - 1);
+                const bool act = o < lw;  // uniform within a group
+                int32_t ss[4] = {0, 0, 0, 0};
+                if (act) {
+                    const int lo = win[2 * o], n = win[2 * o + 1];
+                    // unrolled so that several coefficient loads are in flight
+#pragma unroll 8
+                    for (int t = l; t < n; t += G) accumulate(row[lo + t], kt[(size_t)t * lw + o], ss);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) ss[c] = group_sum(ss[c], G);  // every lane takes part
+                if (act && l == G - 1) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) ss[c] += 1 << (kBits - 1);
+                    dst[o] = pack_clip(ss);
+                }
+            }
+        }
+        __syncthreads();  // the next row overwrites the LDS
     }
 }
 
@@ -167,6 +303,19 @@ int axis_ksize(int in_size, int out_size) {
 
 uint64_t align256(uint64_t v) { return (v + 255) & ~(uint64_t)255; }
 
+// in / out is a power of two of at most 2^25 (the weights then stay exact dyadic
+// fractions, see axis_coeffs)
+bool power_of_two_scale(int in_size, int out_size) {
+    if (in_size % out_size) return false;
+    const int r = in_size / out_size;
+    return (r & (r - 1)) == 0 && r <= (1 << 25);
+}
+
+// Waves of the coefficient launch for one axis (k_mip_coeffs).
+uint32_t coef_waves(int out_size, int ks) {
+    return ks > kResampleWideTaps ? (uint32_t)out_size : (uint32_t)((out_size + 63) / 64);
+}
+
 uint32_t units(uint64_t outputs, int ks) {
     const uint64_t per = ks > kResampleWideTaps ? 4 : 256;
     return (uint32_t)((outputs + per - 1) / per);
@@ -185,6 +334,12 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
         p.lh[m] = m ? std::max(p.lh[m - 1] / 2, 1) : h;
         p.ksx[m] = axis_ksize(w, p.lw[m]);
         p.ksy[m] = axis_ksize(h, p.lh[m]);
+        if (power_of_two_scale(w, p.lw[m])) p.exact_x |= 1u << m;
+        // one lane per output for short windows (consecutive outputs read LDS words
+        // `scale` apart: at most 8-way bank conflicts), a 16-lane group for long
+        // ones (a group reads 16 consecutive words)
+        p.gx[m] = (double)w / (double)p.lw[m] >= 16.0 ? 16 : 1;
+        if (power_of_two_scale(h, p.lh[m])) p.exact_y |= 1u << m;
     }
     uint32_t coef = 0, hu = 0, vu = 0;
     for (int m = 0; m < levels; m++) {
@@ -201,7 +356,7 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
         words += 2ull * p.lw[m];
         p.wy[m] = words;
         words += 2ull * p.lh[m];
-        coef += (uint32_t)(p.lw[m] + p.lh[m]);
+        coef += coef_waves(p.lw[m], p.ksx[m]) + coef_waves(p.lh[m], p.ksy[m]);
         if (p.lw[m] != w) hu += units((uint64_t)p.lw[m] * h, p.ksx[m]);
         if (p.lh[m] != h) vu += units((uint64_t)p.lw[m] * p.lh[m], p.ksy[m]);
     }
@@ -220,11 +375,18 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
 
 hipError_t launch_mips(const MipPlan &p, const uint8_t *d_base, uint8_t *d_ws, hipStream_t stream) {
     if (p.levels <= 1) return hipSuccess;
-    const uint32_t nc = p.coef0[p.levels];
-    hipLaunchKernelGGL(dev::k_mip_coeffs, dim3((nc + 255) / 256), dim3(256), 0, stream, p,
+    const uint32_t nc = p.coef0[p.levels];  // waves
+    hipLaunchKernelGGL(dev::k_mip_coeffs, dim3((nc + 3) / 4), dim3(256), 0, stream, p,
                        reinterpret_cast<int32_t *>(d_ws));
-    if (p.h0[p.levels])
-        hipLaunchKernelGGL(dev::k_mip_pass<false>, dim3(p.h0[p.levels]), dim3(256), 0, stream, p, d_base, d_ws);
+    if (p.h0[p.levels]) {
+        if (p.w <= kRowMaxWidth) {
+            const size_t lds = (size_t)p.w * 4;
+            hipLaunchKernelGGL(dev::k_mip_hrows, dim3((unsigned)std::min(p.h, kRowGrid)), dim3(256), lds, stream, p,
+                               d_base, d_ws);
+        } else {
+            hipLaunchKernelGGL(dev::k_mip_pass<false>, dim3(p.h0[p.levels]), dim3(256), 0, stream, p, d_base, d_ws);
+        }
+    }
     if (p.v0[p.levels])
         hipLaunchKernelGGL(dev::k_mip_pass<true>, dim3(p.v0[p.levels]), dim3(256), 0, stream, p, d_base, d_ws);
     return hipGetLastError();
