@@ -247,6 +247,30 @@ qtx_status qtx_mips_encode_host(qtx_ctx *ctx, const qtx_job *tmpl, const uint8_t
                                 int32_t levels, uint8_t *const *out_blocks, const size_t *out_bytes);
 
 /* ---------------------------------------------------------------------- */
+/* The per-image choices of the reference's `quicktex encode` commands
+ * (quicktex/cli/encode.py), on the device. */
+
+/* *has_alpha = 1 if any texel's alpha is below 255, else 0: the alpha-histogram
+ * test of `encode auto` (cli/encode.py:129-133).  The device form reads a
+ * 16-byte aligned texture on stream and waits for it. */
+qtx_status qtx_has_alpha_device(qtx_ctx *ctx, const uint8_t *d_rgba, int32_t width, int32_t height, void *stream,
+                                int32_t *has_alpha);
+qtx_status qtx_has_alpha_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, int32_t height, int32_t *has_alpha);
+/* `encode auto` for one image (cli/encode.py:103-141): the chain is encoded as
+ * BC3 (bc1 params in FourColor, as BC3Encoder(level) builds its colour encoder)
+ * when the base has alpha below 255, else as BC1 with bc1 params.  *format
+ * receives QTX_FMT_BC3 or QTX_FMT_BC1; out_bytes must hold the BC3 sizes (the
+ * larger).  One upload of the base serves the test and the chain. */
+qtx_status qtx_mips_encode_auto_host(qtx_ctx *ctx, const qtx_bc1_params *bc1, const uint8_t *rgba, int32_t width,
+                                     int32_t height, int32_t levels, uint8_t *const *out_blocks, const size_t *out_bytes,
+                                     int32_t *format);
+/* The normal-map swizzle of `encode bc3nm` (cli/encode.py:52-55): every texel
+ * (R, G, B, A) becomes (255, G, G, R).  Device form in place on stream; host form
+ * from rgba into out (width * height * 4 bytes each). */
+qtx_status qtx_swizzle_bc3nm_device(qtx_ctx *ctx, uint8_t *d_rgba, int32_t width, int32_t height, void *stream);
+qtx_status qtx_swizzle_bc3nm_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, int32_t height, uint8_t *out);
+
+/* ---------------------------------------------------------------------- */
 /* Serving form: a request loop on its own thread.  Requests (encode or
  * decode, any format, size and settings; host buffers) are admitted in
  * arrival order as device memory frees up, grouped into batches -- every

@@ -157,6 +157,12 @@ class Qtx:
         L.qtx_mips_device.argtypes = [vp, vp, i32, i32, i32, PP, vp, sz, vp]
         L.qtx_mips_host.argtypes = [vp, _U8P, i32, i32, i32, PP, ctypes.POINTER(sz)]
         L.qtx_mips_encode_host.argtypes = [vp, JP, _U8P, i32, i32, i32, PP, ctypes.POINTER(sz)]
+        I32P = ctypes.POINTER(i32)
+        L.qtx_has_alpha_device.argtypes = [vp, vp, i32, i32, vp, I32P]
+        L.qtx_has_alpha_host.argtypes = [vp, _U8P, i32, i32, I32P]
+        L.qtx_mips_encode_auto_host.argtypes = [vp, P, _U8P, i32, i32, i32, PP, ctypes.POINTER(sz), I32P]
+        L.qtx_swizzle_bc3nm_device.argtypes = [vp, vp, i32, i32, vp]
+        L.qtx_swizzle_bc3nm_host.argtypes = [vp, _U8P, i32, i32, _U8P]
         self._ctx = None
 
     # -- plumbing -------------------------------------------------------------
@@ -384,6 +390,38 @@ def mips_encode_host(q, tmpl, rgba, levels):
     caps = (ctypes.c_size_t * levels)(*[o.nbytes for o in outs])
     q.check(q.lib.qtx_mips_encode_host(q.ctx, ctypes.byref(tmpl), _ptr(rgba), w, h, levels, ptrs, caps))
     return outs
+
+
+def has_alpha_host(q, rgba):
+    """qtx_has_alpha_host on an (h, w, 4) uint8 array."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    out = ctypes.c_int32(-1)
+    q.check(q.lib.qtx_has_alpha_host(q.ctx, _ptr(rgba), w, h, ctypes.byref(out)))
+    return bool(out.value)
+
+
+def swizzle_bc3nm_host(q, rgba):
+    """qtx_swizzle_bc3nm_host: (R, G, B, A) -> (255, G, G, R) of an (h, w, 4) uint8 array."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    out = np.empty_like(rgba)
+    q.check(q.lib.qtx_swizzle_bc3nm_host(q.ctx, _ptr(rgba), w, h, _ptr(out)))
+    return out
+
+
+def mips_encode_auto_host(q, params, rgba, levels):
+    """qtx_mips_encode_auto_host: (format, encoded levels) of the chain."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    outs = [np.zeros(_nb(lw, lh) * 16, np.uint8) for lw, lh in mip_sizes(w, h, levels)]
+    ptrs = (ctypes.c_void_p * levels)(*[o.ctypes.data for o in outs])
+    caps = (ctypes.c_size_t * levels)(*[o.nbytes for o in outs])
+    fmt = ctypes.c_int32(0)
+    q.check(q.lib.qtx_mips_encode_auto_host(q.ctx, ctypes.byref(params), _ptr(rgba), w, h, levels, ptrs, caps,
+                                            ctypes.byref(fmt)))
+    bpb = 16 if fmt.value == FMT_BC3 else 8
+    return fmt.value, [o[:_nb(lw, lh) * bpb] for o, (lw, lh) in zip(outs, mip_sizes(w, h, levels))]
 
 
 def run_host_batch(q, jobs):

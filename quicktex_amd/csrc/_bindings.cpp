@@ -600,6 +600,63 @@ template <class B> py::list decode_all_as(qtx_job tmpl, const py::list &textures
     return l;
 }
 
+// `encode auto` (quicktex/cli/encode.py:103-141) for one image: (is_bc3, levels).
+py::tuple encode_mips_auto(const BC1Encoder &bc1, py::buffer data, int w, int h, int levels) {
+    if (w <= 0) throw std::invalid_argument("Texture width must be greater than 0");
+    if (h <= 0) throw std::invalid_argument("Texture height must be greater than 0");
+    py::buffer_info info = request_bytes(data, (size_t)w * (size_t)h * 4);
+    // staging sized for BC3; the blocks are copied into textures of the chosen format
+    std::vector<std::vector<uint8_t>> stage;
+    std::vector<uint8_t *> ptrs;
+    std::vector<size_t> caps;
+    std::vector<std::pair<int, int>> sizes;
+    int lw = w, lh = h;
+    for (int m = 0; m < levels; m++) {
+        sizes.emplace_back(lw, lh);
+        stage.emplace_back((size_t)((lw + 3) / 4) * (size_t)((lh + 3) / 4) * 16);
+        ptrs.push_back(stage.back().data());
+        caps.push_back(stage.back().size());
+        lw = std::max(lw / 2, 1);
+        lh = std::max(lh / 2, 1);
+    }
+    int32_t fmt = 0;
+    qtx_status st;
+    {
+        py::gil_scoped_release nogil;
+        st = qtx_mips_encode_auto_host(context(), &bc1.p, static_cast<const uint8_t *>(info.ptr), w, h, levels,
+                                       ptrs.data(), caps.data(), &fmt);
+    }
+    check(st);
+    py::list l;
+    for (int m = 0; m < levels; m++) {
+        if (fmt == QTX_FMT_BC3) {
+            auto t = std::make_unique<BlockTexture<BC3Block>>(sizes[m].first, sizes[m].second);
+            std::memcpy(t->data(), stage[m].data(), t->nbytes());
+            l.append(py::cast(std::move(t)));
+        } else {
+            auto t = std::make_unique<BlockTexture<BC1Block>>(sizes[m].first, sizes[m].second);
+            std::memcpy(t->data(), stage[m].data(), t->nbytes());
+            l.append(py::cast(std::move(t)));
+        }
+    }
+    return py::make_tuple(fmt == QTX_FMT_BC3, l);
+}
+
+py::bytes swizzle_bc3nm(py::buffer data, int w, int h) {
+    if (w <= 0 || h <= 0) throw std::invalid_argument("Texture size must be greater than 0");
+    const size_t n = (size_t)w * (size_t)h * 4;
+    py::buffer_info info = request_bytes(data, n);
+    std::string out(n, '\0');
+    qtx_status st;
+    {
+        py::gil_scoped_release nogil;
+        st = qtx_swizzle_bc3nm_host(context(), static_cast<const uint8_t *>(info.ptr), w, h,
+                                    reinterpret_cast<uint8_t *>(&out[0]));
+    }
+    check(st);
+    return py::bytes(out);
+}
+
 qtx_job decode_template(int format) {
     qtx_job j{};
     j.op = QTX_OP_DECODE;
@@ -796,4 +853,6 @@ PYBIND11_MODULE(_quicktex, m) {
           "height"_a, "requested"_a = 0);
     m.def("_encode_mips", &encode_mips, "encoder"_a, "data"_a, "width"_a, "height"_a, "levels"_a);
     m.def("_decode_all", &decode_all, "decoder"_a, "textures"_a);
+    m.def("_encode_mips_auto", &encode_mips_auto, "bc1_encoder"_a, "data"_a, "width"_a, "height"_a, "levels"_a);
+    m.def("_swizzle_bc3nm", &swizzle_bc3nm, "data"_a, "width"_a, "height"_a);
 }

@@ -59,4 +59,10 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p);
 // Enqueues the coefficient, horizontal and vertical launches (dst pointers set).
 hipError_t launch_mips(const MipPlan &p, const uint8_t *d_base, uint8_t *d_ws, hipStream_t stream);
 
+// Sets *d_flag to 1 if any of the texels has alpha below 255 (the flag must be 0
+// before; the kernel only ever writes 1).  d_rgba 16-byte aligned.
+hipError_t launch_has_alpha(const uint8_t *d_rgba, uint64_t texels, uint32_t *d_flag, hipStream_t stream);
+// (R, G, B, A) -> (255, G, G, R) in place.
+hipError_t launch_swizzle_bc3nm(uint8_t *d_rgba, uint64_t texels, hipStream_t stream);
+
 }  // namespace qtx

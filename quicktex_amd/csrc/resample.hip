@@ -280,6 +280,38 @@ __global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-texel helpers of the DDS encoder's callers (quicktex/cli/encode.py).
+
+// *flag = 1 when a texel's alpha is below 255 (the alpha histogram test of
+// `encode auto`, cli/encode.py:129-133: a count in any bin but 255).  16 bytes
+// per lane; the tail texels one by one.  Writers only ever store 1.
+__global__ __launch_bounds__(256) void k_has_alpha(const uint8_t *__restrict__ rgba, uint64_t texels,
+                                                   uint32_t *__restrict__ flag) {
+    const uint64_t quads = texels / 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    bool below = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < quads; i += stride) {
+        const uint4 q = reinterpret_cast<const uint4 *>(rgba)[i];
+        below |= (q.x & q.y & q.z & q.w) < 0xFF000000u;
+    }
+    const uint64_t t = quads * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < texels) below |= reinterpret_cast<const uint32_t *>(rgba)[t] < 0xFF000000u;
+    if (__ballot(below) != 0 && (threadIdx.x & 63) == 0) *flag = 1u;
+}
+
+// The normal-map swizzle of `encode bc3nm` (cli/encode.py:52-55): each texel
+// (R, G, B, A) becomes (255, G, G, R).
+__global__ __launch_bounds__(256) void k_swizzle_bc3nm(uint8_t *__restrict__ rgba, uint64_t texels) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t *px = reinterpret_cast<uint32_t *>(rgba);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < texels; i += stride) {
+        const uint32_t v = px[i];
+        const uint32_t r = v & 0xFFu, g = (v >> 8) & 0xFFu;
+        px[i] = 0xFFu | (g << 8) | (g << 16) | (r << 24);
+    }
+}
+
 }  // namespace dev
 
 int mip_level_count(int32_t w, int32_t h, int32_t requested) {
@@ -371,6 +403,19 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
         }
     }
     return (size_t)bytes;
+}
+
+hipError_t launch_has_alpha(const uint8_t *d_rgba, uint64_t texels, uint32_t *d_flag, hipStream_t stream) {
+    const uint64_t quads = texels / 4;
+    const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((quads + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(dev::k_has_alpha, dim3(grid), dim3(256), 0, stream, d_rgba, texels, d_flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_swizzle_bc3nm(uint8_t *d_rgba, uint64_t texels, hipStream_t stream) {
+    const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((texels + 255) / 256, 1), 8192);
+    hipLaunchKernelGGL(dev::k_swizzle_bc3nm, dim3(grid), dim3(256), 0, stream, d_rgba, texels);
+    return hipGetLastError();
 }
 
 hipError_t launch_mips(const MipPlan &p, const uint8_t *d_base, uint8_t *d_ws, hipStream_t stream) {

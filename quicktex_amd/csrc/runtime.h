@@ -22,6 +22,7 @@ struct qtx_ctx {
     size_t in_cap = 0;
     uint8_t *d_out = nullptr;
     size_t out_cap = 0;
+    uint32_t *d_flag = nullptr;  // result word of the alpha test
     std::mutex mu;
     // Work-queue words of the queued BC1/BC3 lane kernel: kQueueSlots sets, each
     // lent to one launch at a time.  A set is free again once the event recorded

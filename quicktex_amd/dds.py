@@ -12,6 +12,9 @@ work runs, not in the bytes:
   the levels equal Pillow's BILINEAR resize (image_utils.resize_no_premultiply)
   and the blocks equal per-level encoder.encode calls.
 * :meth:`DDSFile.decode_all` decodes every level in one batched launch.
+* :func:`encode_auto` and :func:`swizzle_bc3nm` are the per-image choices of the
+  reference's ``quicktex encode auto`` / ``encode bc3nm`` commands
+  (cli/encode.py:52-55, 103-141), with the alpha test and the swizzle on the GPU.
 """
 from __future__ import annotations
 
@@ -187,3 +190,57 @@ def encode(image, encoder, four_cc: str, mip_count: typing.Optional[int] = None)
     dds.pf_flags = PFFlags.FOURCC
     dds.four_cc = four_cc
     return dds
+
+
+def encode_auto(image, level: int = 18, black: bool = False, threecolor: bool = True,
+                mip_count: typing.Optional[int] = None) -> DDSFile:
+    """``quicktex encode auto`` for one image (quicktex/cli/encode.py:103-141; the
+    command's file handling and flip stay with the caller): BC3 (DXT5) when any
+    texel's alpha is below 255, else BC1 (DXT1) in the colour mode the flags
+    choose.  The alpha test, the mip chain and the encode run on the GPU after one
+    upload of the image."""
+    mode_enum = bc1.BC1Encoder.ColorMode
+    if not threecolor:
+        color_mode = mode_enum.FourColor
+    elif not black:
+        color_mode = mode_enum.ThreeColor
+    else:
+        color_mode = mode_enum.ThreeColorBlack
+    encoder = bc1.BC1Encoder(level, color_mode)
+
+    mode = 'RGBA' if 'A' in image.mode else 'RGBX'
+    image.apply_transparency()
+    image = image.convert(mode)
+    sizes = mip_sizes(image.size, mip_count)
+    is_bc3, textures = _native()._encode_mips_auto(encoder, image.tobytes('raw', mode), image.size[0], image.size[1],
+                                                   len(sizes))
+    dds = DDSFile()
+    dds.textures = list(textures)
+    dds.flags = DDSFlags.TEXTURE | DDSFlags.LINEAR_SIZE
+    caps0 = Caps0.TEXTURE
+    if len(sizes) > 1:
+        dds.flags |= DDSFlags.MIPMAPCOUNT
+        caps0 |= Caps0.MIPMAP | Caps0.COMPLEX
+    dds.caps = (caps0, 0, 0, 0)
+    dds.mipmap_count = len(sizes)
+    dds.pitch = dds.textures[0].nbytes
+    dds.size = dds.textures[0].size
+    dds.pf_flags = PFFlags.FOURCC
+    dds.four_cc = 'DXT5' if is_bc3 else 'DXT1'
+    dds.format = dds_formats[1] if is_bc3 else dds_formats[0]
+    return dds
+
+
+def swizzle_bc3nm(image):
+    """The normal-map swizzle of ``quicktex encode bc3nm`` (quicktex/cli/encode.py:52-55)
+    on the GPU: an RGBA image (255, band 1, band 1, band 0) of an image of two or
+    more bands."""
+    import numpy as np
+    from PIL import Image
+    bands = image.split()
+    w, h = image.size
+    packed = np.zeros((h, w, 4), np.uint8)
+    packed[..., 0] = np.asarray(bands[0])
+    packed[..., 1] = np.asarray(bands[1])
+    out = _native()._swizzle_bc3nm(packed.reshape(-1), w, h)
+    return Image.frombytes('RGBA', (w, h), out)
