@@ -64,6 +64,32 @@ __device__ __forceinline__ uint32_t prnd565(float r, float g, float b) {
     return ep_make((uint32_t)ir, (uint32_t)ig, (uint32_t)ib);
 }
 
+// PreciseRound565 of both endpoints of a least-squares solve.  Every component
+// there is bounded far inside the int range: the solve's coefficients are an
+// adjugate of integer weight sums (entries <= 144) over a non-zero integer
+// determinant, times 3/255 or 2/255, so |coefficient| < 2, and the texel sums are
+// <= 3 x 16 x 255, so |component| < 2^17.  The truncating conversion then needs
+// no out-of-range case, and the six rounding-point reads are issued together.
+__device__ __forceinline__ uint32_t prnd565_ls(float lr, float lg, float lb, float hr, float hg, float hb) {
+    const int ilr = clampi((int)(lr * 31.0f), 0, 31), ilg = clampi((int)(lg * 63.0f), 0, 63);
+    const int ilb = clampi((int)(lb * 31.0f), 0, 31), ihr = clampi((int)(hr * 31.0f), 0, 31);
+    const int ihg = clampi((int)(hg * 63.0f), 0, 63), ihb = clampi((int)(hb * 31.0f), 0, 31);
+    const float mlr = LL().mid5[ilr], mlg = LL().mid6[ilg], mlb = LL().mid5[ilb];
+    const float mhr = LL().mid5[ihr], mhg = LL().mid6[ihg], mhb = LL().mid5[ihb];
+    const uint32_t lo = ep_make((uint32_t)(ilr + (lr > mlr)), (uint32_t)(ilg + (lg > mlg)), (uint32_t)(ilb + (lb > mlb)));
+    const uint32_t hi = ep_make((uint32_t)(ihr + (hr > mhr)), (uint32_t)(ihg + (hg > mhg)), (uint32_t)(ihb + (hb > mhb)));
+    return lo | (hi << 16);
+}
+
+// -v, opaque to the optimiser: 2 p.c - c.c is then written (p.c << 1) + (-c.c), one
+// v_lshl_add_u32, where the folded form 2 p.c - c.c costs a shift and a subtract.
+__device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
+    uint32_t n = 0u - v;
+    asm("" : "+v"(n));
+    return n;
+}
+__device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
+
 // FindEndpointsSingleColor(result, color, is_3color) (BC1Encoder.cpp:443-456): the pair
 // from the match tables (the 3-colour tables equal the 4-colour ones); err gets the
 // table error.
@@ -354,11 +380,13 @@ __device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], con
     }
     // Full: every candidate, ties to the later one only for index 3
     const bool three = M == QTX_COLOR_THREE;
+    const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
+    const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const uint32_t p = px[i];
-        const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
-        const int u2 = 2 * (int)dot4(p, c[2]) - cc2, u3 = 2 * (int)dot4(p, c[3]) - cc3;
+        const int u0 = twice_plus(dot4(p, c[0]), n0), u1 = twice_plus(dot4(p, c[1]), n1);
+        const int u2 = twice_plus(dot4(p, c[2]), n2), u3 = twice_plus(dot4(p, c[3]), n3);
         int bu = u0;
         uint32_t s = 0;
         if (u1 > bu) { bu = u1; s = 1; }
@@ -378,29 +406,30 @@ __device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const 
                                                 uint32_t ep) {
     uint32_t c[4];
     palette_pair<INTERP>(ep, (M & 0xF) == 3, c);
-    const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
-    const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
     int tot = 0;
     if (error_mode == QTX_ERROR_FULL) {
+        const uint32_t n0 = neg_opaque(dot4(c[0], c[0])), n1 = neg_opaque(dot4(c[1], c[1]));
+        const uint32_t n2 = neg_opaque(dot4(c[2], c[2]));
         if (M == QTX_COLOR_THREE) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t p = px[i];
-                const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
-                const int u2 = 2 * (int)dot4(p, c[2]) - cc2;
-                tot += max(max(u0, u1), u2);
+                tot += max(max(twice_plus(dot4(p, c[0]), n0), twice_plus(dot4(p, c[1]), n1)), twice_plus(dot4(p, c[2]), n2));
             }
         } else {
+            const uint32_t n3 = neg_opaque(dot4(c[3], c[3]));
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t p = px[i];
-                const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
-                const int u2 = 2 * (int)dot4(p, c[2]) - cc2, u3 = 2 * (int)dot4(p, c[3]) - cc3;
+                const int u0 = twice_plus(dot4(p, c[0]), n0), u1 = twice_plus(dot4(p, c[1]), n1);
+                const int u2 = twice_plus(dot4(p, c[2]), n2), u3 = twice_plus(dot4(p, c[3]), n3);
                 tot += max(max(u0, u1), max(u2, u3));
             }
         }
         return bk.pp - (uint32_t)tot;
     }
+    const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
+    const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
     if (error_mode == QTX_ERROR_CHECK2) {
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
@@ -468,9 +497,8 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uin
     const float a0 = (m3 * 1.0f) * scale, a1 = (m1 * -1.0f) * scale, a2 = (m1 * -1.0f) * scale, a3 = (m0 * 1.0f) * scale;
     const float q0 = (float)(qa & 0xFFFFu), q1 = (float)(qa >> 16), q2 = (float)qb;
     const float r0 = (float)(m.s0 * 3) - q0, r1 = (float)(m.s1 * 3) - q1, r2 = (float)(m.s2 * 3) - q2;
-    const uint32_t lo = prnd565((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2));
-    const uint32_t hi = prnd565((a2 * q0) + (a3 * r0), (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2));
-    return lo | (hi << 16);
+    return prnd565_ls((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2), (a2 * q0) + (a3 * r0),
+                      (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2));
 }
 
 // RefineBlockLS<M> (BC1Encoder.cpp:852-875)
@@ -573,13 +601,31 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     touch(px);
     const uint32_t q_total = exhaustive ? (four ? (uint32_t)kOrders4 : (uint32_t)kOrders3) : orderings;
     const uint16_t *best_row = four ? &T->best4[start * kBestOrders4] : &T->best3[start * kBestOrders3];
+    const uint32_t *levels = four ? T->levels4 : T->levels3;
+    const float4 *factors = four ? reinterpret_cast<const float4 *>(T->factors4) : reinterpret_cast<const float4 *>(T->factors3);
+    // The table reads run ahead of the trials: the ordering of trial q + 2 and the
+    // levels and factors of trial q + 1 are loaded while trial q computes.  Reads
+    // past the last trial fetch the last ordering again (always in bounds).
+    const uint32_t q_last = q_total - 1;
+    uint32_t h_next = exhaustive ? min(1u, q_last) : best_row[min(1u, q_last)];
+    uint32_t h_cur = exhaustive ? 0u : best_row[0];
+    uint32_t lv_next = levels[h_cur];
+    float4 fm_next = factors[h_cur];
+    // wait for the first trial's tables here, so that no load of the loop's entry
+    // is still counted as outstanding inside the loop (where the wait would then
+    // also cover the loads issued ahead)
+    asm volatile("" ::"v"(lv_next), "v"(fm_next.x), "v"(fm_next.y), "v"(fm_next.z), "v"(fm_next.w));
     // trials keep (error, endpoints, single-colour flag); the winner's selectors
     // are computed once at the end
     uint32_t b_err = orig.err, b_ep = orig.ep;
     bool b_single = false, improved = false;
     for (uint32_t q = 0; q < q_total; q++) {
-        const uint32_t hash = exhaustive ? q : best_row[q];
-        const uint32_t lv = four ? T->levels4[hash] : T->levels3[hash];
+        const uint32_t lv = lv_next;
+        const float4 fm = fm_next;
+        lv_next = levels[h_next];
+        fm_next = factors[h_next];
+        const uint32_t q2 = min(q + 2u, q_last);
+        h_next = exhaustive ? q2 : best_row[q2];
         uint32_t t_err, t_ep;
         if (lv >> 24) {
             const Res t = single_color_px<INTERP>(px, bk, orig.ep, m.avg, !four);
@@ -593,15 +639,11 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
             const uint2 v2 = l2 ? LL().sums[l2 - 1][threadIdx.x] : make_uint2(0, 0);
             const uint2 v3 = (four && l3) ? LL().sums[l3 - 1][threadIdx.x] : make_uint2(0, 0);
             const uint32_t qa = v1.x + v2.x + v3.x, qb = v1.y + v2.y + v3.y;
-            const float4 fm = four ? *reinterpret_cast<const float4 *>(T->factors4[hash])
-                                   : *reinterpret_cast<const float4 *>(T->factors3[hash]);
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
-            const uint32_t lo = prnd565((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
-                                        (fm.x * q00_2) + (fm.y * q10_2));
-            const uint32_t hi = prnd565((fm.z * q00_0) + (fm.w * q10_0), (fm.z * q00_1) + (fm.w * q10_1),
-                                        (fm.z * q00_2) + (fm.w * q10_2));
-            t_ep = lo | (hi << 16);
+            t_ep = prnd565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
+                              (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
+                              (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2));
             t_err = trial_error<INTERP>(px, bk, M, error_mode, t_ep);
         }
         if (t_err < b_err) {
