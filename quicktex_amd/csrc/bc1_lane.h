@@ -28,16 +28,37 @@ namespace qtx {
 namespace dev {
 namespace lane {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;       // workgroup of the static and batch lane kernels
+constexpr int kQueueThreads = 768;  // workgroup of the queued kernel: one per CU, 3 waves per SIMD
 
-// Per-workgroup LDS of the lane kernels.
-struct LaneLds {
+// Per-workgroup LDS of the lane kernels: this head, then (with palette tables,
+// PAL) the tables, then the cluster-fit prefix sums, [16][workgroup size]:
+// sums[k][t] = prefix sum k+1 of thread t's sorted texels (r | g << 16, b).
+struct LaneHead {
     uint32_t match[2][256];  // this launch's interpolator, 5- and 6-bit
     float mid5[32], mid6[64];
-    uint2 sums[16][kThreads];  // sums[k][t]: prefix sum k+1 of thread t's sorted texels (r | g << 16, b)
-    uint32_t wave_state[kThreads / 64][4];  // the queued kernel's per-wave dealing state
+    uint32_t wave_state[kQueueThreads / 64][4];  // the queued kernel's per-wave dealing state
 };
-__device__ __forceinline__ LaneLds &LL() { return *reinterpret_cast<LaneLds *>(qtx_lds_raw); }
+// Palette tables of this launch's interpolator: entry lo | hi << bits of a 5- or
+// 6-bit channel pair holds that channel of the four palette colours in the
+// encoder's internal order (bytes 0-3), for 4-colour and for 3-colour blocks.
+// Red and blue share the 5-bit tables.
+constexpr int kPal5 = 32 * 32, kPal6 = 64 * 64;
+struct LanePal {
+    uint32_t four5[kPal5], four6[kPal6], three5[kPal5], three6[kPal6];
+};
+template <bool PAL> struct LaneLayout {
+    static constexpr int kGroup = PAL ? kQueueThreads : kThreads;
+    static constexpr size_t kSumsOffset = sizeof(LaneHead) + (PAL ? sizeof(LanePal) : 0);
+    static constexpr size_t kBytes = kSumsOffset + sizeof(uint2) * 16 * kGroup;
+};
+static_assert(sizeof(LaneHead) % 16 == 0 && sizeof(LanePal) % 16 == 0, "LDS sections stay 16-byte aligned");
+__device__ __forceinline__ LaneHead &LL() { return *reinterpret_cast<LaneHead *>(qtx_lds_raw); }
+__device__ __forceinline__ LanePal &LP() { return *reinterpret_cast<LanePal *>(qtx_lds_raw + sizeof(LaneHead)); }
+// this thread's prefix sum k + 1
+template <bool PAL> __device__ __forceinline__ uint2 &lane_sum(uint32_t k) {
+    return reinterpret_cast<uint2 *>(qtx_lds_raw + LaneLayout<PAL>::kSumsOffset)[k * LaneLayout<PAL>::kGroup + threadIdx.x];
+}
 
 // Optimisation barrier on the texel registers (emits no instruction): values the
 // compiler derived from the texels before it are not carried past it, which keeps
@@ -47,10 +68,48 @@ __device__ __forceinline__ void touch(uint32_t (&px)[16]) {
     for (int i = 0; i < 16; i++) asm volatile("" : "+v"(px[i]));
 }
 
-template <int INTERP> __device__ __forceinline__ void load_tables(const DeviceTables *T) {
+template <int INTERP, bool PAL> __device__ __forceinline__ void load_tables(const DeviceTables *T) {
     for (int i = (int)threadIdx.x; i < 512; i += (int)blockDim.x) LL().match[i >> 8][i & 255] = T->match[INTERP][i >> 8][i & 255];
     if (threadIdx.x < 32) LL().mid5[threadIdx.x] = T->mid5[threadIdx.x];
     if (threadIdx.x < 64) LL().mid6[threadIdx.x] = T->mid6[threadIdx.x];
+    if (PAL) {
+        // computed by palette() itself, channel by channel (its channels are
+        // independent), so a lookup equals the arithmetic exactly
+        for (int i = (int)threadIdx.x; i < 2 * (kPal5 + kPal6); i += (int)blockDim.x) {
+            const bool three = i >= kPal5 + kPal6;
+            const int j = three ? i - (kPal5 + kPal6) : i;
+            const bool six = j >= kPal5;
+            const uint32_t k = (uint32_t)(six ? j - kPal5 : j);
+            const uint32_t lo = six ? k & 63u : k & 31u, hi = six ? k >> 6 : k >> 5;
+            uint32_t c[4];
+            palette<INTERP>(six ? rgb(0, lo, 0) : rgb(lo, 0, 0), six ? rgb(0, hi, 0) : rgb(hi, 0, 0), three, c);
+            const uint32_t e = chn(c[0], six) | (chn(c[1], six) << 8) | (chn(c[2], six) << 16) | (chn(c[3], six) << 24);
+            uint32_t *t = three ? (six ? LP().three6 : LP().three5) : (six ? LP().four6 : LP().four5);
+            t[k] = e;
+        }
+    }
+}
+
+// The palette of endpoints ep (internal order, as palette_pair), from the LDS
+// tables when PAL: three table reads and a byte transpose.
+template <int INTERP, bool PAL> __device__ __forceinline__ void lane_palette(uint32_t ep, bool three, uint32_t c[4]) {
+    if (!PAL) {
+        palette_pair<INTERP>(ep, three, c);
+        return;
+    }
+    const uint32_t *t5 = three ? LP().three5 : LP().four5;
+    const uint32_t *t6 = three ? LP().three6 : LP().four6;
+    const uint32_t r = t5[((ep >> 11) & 31u) | ((ep >> 22) & 0x3E0u)];  // low r | high r << 5
+    const uint32_t g = t6[((ep >> 5) & 63u) | ((ep >> 15) & 0xFC0u)];   // low g | high g << 6
+    const uint32_t b = t5[(ep & 31u) | ((ep >> 11) & 0x3E0u)];          // low b | high b << 5
+    // c[k] = r.byte k | g.byte k << 8 | b.byte k << 16 (v_perm_b32: selector 0-3 picks
+    // a byte of the second operand, 4-7 of the first, 12 gives zero)
+    const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);  // r0 g0 r1 g1
+    const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);  // r2 g2 r3 g3
+    c[0] = __builtin_amdgcn_perm(b, rg01, 0x0C040100u);
+    c[1] = __builtin_amdgcn_perm(b, rg01, 0x0C050302u);
+    c[2] = __builtin_amdgcn_perm(b, rg23, 0x0C060100u);
+    c[3] = __builtin_amdgcn_perm(b, rg23, 0x0C070302u);
 }
 
 // Color::PreciseRound565 (quicktex/Color.cpp:163-183)
@@ -322,11 +381,11 @@ struct Blk {
 // FindSelectors<M> (BC1Encoder.cpp:679-779) for endpoints ep: returns the block
 // error (0 for ErrorMode::None, which computes none), sel gets the selectors
 // (internal order).  |p - c|^2 = p.p - (2 p.c - c.c), with v_dot4_u32_u8.
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
                                                    uint32_t ep, uint32_t &sel) {
     uint32_t c[4];
-    palette_pair<INTERP>(ep, (M & 0xF) == 3, c);
+    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
     const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
     const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
     int tot = 0;
@@ -401,11 +460,11 @@ __device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], con
 
 // The error find_selectors returns, without the selectors: what cluster-fit and
 // endpoint-search trials need (the winner's selectors are computed once).
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
                                                 uint32_t ep) {
     uint32_t c[4];
-    palette_pair<INTERP>(ep, (M & 0xF) == 3, c);
+    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
     int tot = 0;
     if (error_mode == QTX_ERROR_FULL) {
         const uint32_t n0 = neg_opaque(dot4(c[0], c[0])), n1 = neg_opaque(dot4(c[1], c[1]));
@@ -450,17 +509,17 @@ __device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const 
         return bk.pp - (uint32_t)tot;
     }
     uint32_t sel;
-    return find_selectors<INTERP>(px, bk, M, error_mode, ep, sel);
+    return find_selectors<INTERP, PAL>(px, bk, M, error_mode, ep, sel);
 }
 
 // FindEndpointsSingleColor(result, pixels, color, is_3color) (BC1Encoder.cpp:458-471):
 // match-table endpoints, error against colour 2 of the endpoints held on entry
 // (alpha included: that colour's alpha is 255), all selectors 1.
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ Res single_color_px(const uint32_t (&px)[16], const Blk &bk, uint32_t ep_in, uint32_t color,
                                                bool three) {
     uint32_t c[4];
-    palette_pair<INTERP>(ep_in, three, c);
+    lane_palette<INTERP, PAL>(ep_in, three, c);
     const uint32_t qc = c[1];
     uint32_t pq = 0;
 #pragma unroll
@@ -502,7 +561,7 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uin
 }
 
 // RefineBlockLS<M> (BC1Encoder.cpp:852-875)
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ Res refine_block_ls(uint32_t (&px)[16], const Blk &bk, const Metrics &m, int M,
                                                int error_mode, int passes, Res r) {
     for (int pass = 0; pass < passes; pass++) {
@@ -511,10 +570,10 @@ __device__ __forceinline__ Res refine_block_ls(uint32_t (&px)[16], const Blk &bk
         const uint32_t ep = ls_from_pixels(px, r.sel, m);
         touch(px);
         if (ep == 0xFFFFFFFFu) {
-            t = single_color_px<INTERP>(px, bk, r.ep, m.avg, M != QTX_COLOR_FOUR);
+            t = single_color_px<INTERP, PAL>(px, bk, r.ep, m.avg, M != QTX_COLOR_FOUR);
         } else {
             t.ep = ep;
-            t.err = find_selectors<INTERP>(px, bk, M, error_mode, ep, t.sel);
+            t.err = find_selectors<INTERP, PAL>(px, bk, M, error_mode, ep, t.sel);
             t.mode = M;
         }
         if (t.ep == r.ep) break;
@@ -552,7 +611,7 @@ __device__ __forceinline__ void sort16(uint32_t (&k)[16], uint32_t (&p)[16]) {
 
 // Cluster fit, RefineBlockCF<M> (BC1Encoder.cpp:877-928).  Trials are independent
 // (each starts from orig); the first strictly lowest error wins, as in the loop.
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, const Metrics &m,
                                            const DeviceTables *__restrict__ T, int M, int error_mode, uint32_t orderings,
                                            bool exhaustive, const Res orig) {
@@ -593,7 +652,7 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     for (int i = 0; i < 16; i++) {
         sa += chn(sp[i], 0) | (chn(sp[i], 1) << 16);
         sb += chn(sp[i], 2);
-        LL().sums[i][threadIdx.x] = make_uint2(sa, sb);
+        lane_sum<PAL>((uint32_t)i) = make_uint2(sa, sb);
     }
     const float den = four ? 3.0f : 2.0f;
     const float s16_0 = (float)(sa & 0xFFFFu), s16_1 = (float)(sa >> 16), s16_2 = (float)sb;
@@ -628,23 +687,23 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
         h_next = exhaustive ? q2 : best_row[q2];
         uint32_t t_err, t_ep;
         if (lv >> 24) {
-            const Res t = single_color_px<INTERP>(px, bk, orig.ep, m.avg, !four);
+            const Res t = single_color_px<INTERP, PAL>(px, bk, orig.ep, m.avg, !four);
             t_err = t.err;
             t_ep = t.ep;
         } else {
             // RefineEndpointsLS(result, sums, matrix, hash) (:827-850):
             // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10, exact integers
             const uint32_t l1 = lv & 0xFF, l2 = (lv >> 8) & 0xFF, l3 = (lv >> 16) & 0xFF;
-            const uint2 v1 = l1 ? LL().sums[l1 - 1][threadIdx.x] : make_uint2(0, 0);
-            const uint2 v2 = l2 ? LL().sums[l2 - 1][threadIdx.x] : make_uint2(0, 0);
-            const uint2 v3 = (four && l3) ? LL().sums[l3 - 1][threadIdx.x] : make_uint2(0, 0);
+            const uint2 v1 = l1 ? lane_sum<PAL>(l1 - 1) : make_uint2(0, 0);
+            const uint2 v2 = l2 ? lane_sum<PAL>(l2 - 1) : make_uint2(0, 0);
+            const uint2 v3 = (four && l3) ? lane_sum<PAL>(l3 - 1) : make_uint2(0, 0);
             const uint32_t qa = v1.x + v2.x + v3.x, qb = v1.y + v2.y + v3.y;
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
             t_ep = prnd565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
                               (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
                               (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2));
-            t_err = trial_error<INTERP>(px, bk, M, error_mode, t_ep);
+            t_err = trial_error<INTERP, PAL>(px, bk, M, error_mode, t_ep);
         }
         if (t_err < b_err) {
             b_err = t_err;
@@ -657,14 +716,14 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     if (!improved) return orig;
     if (b_single) return Res{b_ep, 0x55555555u, b_err, four ? QTX_COLOR_FOUR : QTX_COLOR_THREE};
     Res best{b_ep, 0u, b_err, M};
-    (void)find_selectors<INTERP>(px, bk, M, error_mode, b_ep, best.sel);
+    (void)find_selectors<INTERP, PAL>(px, bk, M, error_mode, b_ep, best.sel);
     return best;
 }
 
 // EndpointSearch (BC1Encoder.cpp:930-996), the reference's sequential loop: round i
 // moves the low (i & 16) or high endpoint by voxel i & 15, skips the reverse of the
 // last improving move, and stops 33 rounds after the last improvement.
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const Blk &bk, uint32_t rounds,
                                                int four_error_mode, Res r) {
     const bool three = r.mode == QTX_COLOR_THREE || r.mode == QTX_COLOR_THREE_BLACK;
@@ -680,10 +739,10 @@ __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const B
                                     (uint32_t)clampi((int)ep_g(e) + vox_d(kVoxGp, v), 0, 63),
                                     (uint32_t)clampi((int)ep_b(e) + vox_d(kVoxBp, v), 0, 31));
         const uint32_t ep = (i & 16u) ? ((r.ep & 0xFFFF0000u) | ne) : ((r.ep & 0xFFFFu) | (ne << 16));
-        const uint32_t err = trial_error<INTERP>(px, bk, M, emode, ep);
+        const uint32_t err = trial_error<INTERP, PAL>(px, bk, M, emode, ep);
         if (err < r.err) {
             r = Res{ep, 0u, err, M};
-            (void)find_selectors<INTERP>(px, bk, M, emode, ep, r.sel);
+            (void)find_selectors<INTERP, PAL>(px, bk, M, emode, ep, r.sel);
             forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
             prev = i;
         }
@@ -699,7 +758,7 @@ __host__ __device__ constexpr bool lane_settings(int exhaustive) { return exhaus
 
 // BC1Encoder::EncodeBlock (BC1Encoder.cpp:261-342) for one block (texels RGBA8,
 // index x + 4y), any settings.
-template <int INTERP>
+template <int INTERP, bool PAL>
 __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC1Settings &s,
                                             const DeviceTables *__restrict__ T) {
     bool solid = true;
@@ -726,8 +785,8 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
         const uint32_t ep = find_endpoints(px, m, ep_mode, false, s.power_iterations);
         touch(px);
         Res t{ep, 0u, 0u, QTX_COLOR_FOUR};
-        t.err = find_selectors<INTERP>(px, bk, QTX_COLOR_FOUR, error_mode, ep, t.sel);
-        t = refine_block_ls<INTERP>(px, bk, m, QTX_COLOR_FOUR, error_mode, s.ls_passes, t);
+        t.err = find_selectors<INTERP, PAL>(px, bk, QTX_COLOR_FOUR, error_mode, ep, t.sel);
+        t = refine_block_ls<INTERP, PAL>(px, bk, m, QTX_COLOR_FOUR, error_mode, s.ls_passes, t);
         if (t.err < result.err) {
             result = t;
             orig = Res{ep, 0u, 0xFFFFFFFFu, 0};
@@ -736,16 +795,16 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
     touch(px);
     if (result.err > 0) {
         for (int it = 0; it < s.cf_passes; it++)
-            result = cluster_fit<INTERP>(px, bk, m, T, QTX_COLOR_FOUR, s.error_mode, s.orderings4, s.exhaustive, result);
+            result = cluster_fit<INTERP, PAL>(px, bk, m, T, QTX_COLOR_FOUR, s.error_mode, s.orderings4, s.exhaustive, result);
     }
     touch(px);
     if (result.err > 0 && (s.color_mode & QTX_COLOR_THREE)) {
         Res t{orig.ep, 0u, 0u, QTX_COLOR_THREE};
-        t.err = find_selectors<INTERP>(px, bk, QTX_COLOR_THREE, QTX_ERROR_FULL, orig.ep, t.sel);
-        t = refine_block_ls<INTERP>(px, bk, m, QTX_COLOR_THREE, QTX_ERROR_FULL, s.ls_passes, t);
+        t.err = find_selectors<INTERP, PAL>(px, bk, QTX_COLOR_THREE, QTX_ERROR_FULL, orig.ep, t.sel);
+        t = refine_block_ls<INTERP, PAL>(px, bk, m, QTX_COLOR_THREE, QTX_ERROR_FULL, s.ls_passes, t);
         if (t.err > 0) {
             for (int it = 0; it < s.cf_passes; it++)
-                t = cluster_fit<INTERP>(px, bk, m, T, QTX_COLOR_THREE, QTX_ERROR_FULL, s.orderings3, s.exhaustive, t);
+                t = cluster_fit<INTERP, PAL>(px, bk, m, T, QTX_COLOR_THREE, QTX_ERROR_FULL, s.orderings3, s.exhaustive, t);
         }
         if (t.err < result.err) result = t;
     }
@@ -757,13 +816,13 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
         const uint32_t ep = find_endpoints(px, mnb, QTX_EP_PCA, true, s.power_iterations);
         touch(px);
         Res t{ep, 0u, 0u, QTX_COLOR_THREE_BLACK};
-        t.err = find_selectors<INTERP>(px, bk, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, ep, t.sel);
-        t = refine_block_ls<INTERP>(px, bk, mnb, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, s.ls_passes, t);
+        t.err = find_selectors<INTERP, PAL>(px, bk, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, ep, t.sel);
+        t = refine_block_ls<INTERP, PAL>(px, bk, mnb, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, s.ls_passes, t);
         if (t.err < result.err) result = t;
     }
     if (result.err > 0 && s.search_rounds > 0) {
         touch(px);
-        result = endpoint_search<INTERP>(px, bk, s.search_rounds, s.error_mode, result);
+        result = endpoint_search<INTERP, PAL>(px, bk, s.search_rounds, s.error_mode, result);
     }
     return write_block(result);
 }

@@ -161,7 +161,7 @@ __device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rg
                                                   const BC1Settings &s, bool bc3) {
     uint32_t px[16];
     load_block16(rgba, w, h, bw, gid, px);
-    store_lane_block(out, (uint32_t)gid, px, lane::encode_bc1<INTERP>(px, s, T), bc3);
+    store_lane_block(out, (uint32_t)gid, px, lane::encode_bc1<INTERP, false>(px, s, T), bc3);
 }
 
 // BC1/BC3 encode, one block per lane, for settings without endpoint search.
@@ -169,7 +169,7 @@ template <int INTERP, bool BC3>
 __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
                                                   BC1Settings s) {
-    lane::load_tables<INTERP>(T);
+    lane::load_tables<INTERP, false>(T);
     __syncthreads();
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (gid < nblocks) encode_lane_block<INTERP>(rgba, w, h, bw, gid, out, T, s, BC3);
@@ -219,12 +219,12 @@ __device__ __forceinline__ void retire_lane_queue(uint32_t *queue, uint32_t wave
 }
 
 template <int INTERP, bool BC3>
-__global__ __launch_bounds__(256, 3) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
+__global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
                                                         uint32_t nblocks, uint8_t *__restrict__ out,
                                                         const DeviceTables *__restrict__ T, BC1Settings s,
                                                         uint32_t *__restrict__ queue) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    lane::load_tables<INTERP>(T);
+    lane::load_tables<INTERP, true>(T);
     // the wave's current range and the number of ranges it found used up live in
     // LDS between refills, so that they hold no registers through the searches
     uint32_t *const ws = lane::LL().wave_state[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256, 3) void k_bc1_lane_queue(const uint8_t *__rest
             asm volatile("" ::: "memory");
         }
         if (__ballot(gid != kNone) == 0) break;
-        if (gid != kNone) store_lane_block(out, gid, px, lane::encode_bc1<INTERP>(px, s, T), BC3);
+        if (gid != kNone) store_lane_block(out, gid, px, lane::encode_bc1<INTERP, true>(px, s, T), BC3);
     }
     retire_lane_queue(queue, gridDim.x * (blockDim.x / 64));
 }
@@ -535,7 +535,7 @@ __device__ __forceinline__ const BatchJob &unit_job(const BatchJob *__restrict__
 template <int INTERP>
 __global__ __launch_bounds__(256, 4) void k_lane_batch(const BatchJob *__restrict__ jobs, int njobs,
                                                     const DeviceTables *__restrict__ T) {
-    lane::load_tables<INTERP>(T);
+    lane::load_tables<INTERP, false>(T);
     __syncthreads();
     const uint32_t u = blockIdx.x;
     const BatchJob &J = unit_job(jobs, njobs, u);
@@ -585,7 +585,8 @@ __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict_
 namespace {
 constexpr int kThreads = 256;
 constexpr size_t kBc1Lds = sizeof(dev::Lds);
-constexpr size_t kLaneLds = sizeof(dev::lane::LaneLds);
+constexpr size_t kLaneLds = dev::lane::LaneLayout<false>::kBytes;
+constexpr size_t kLaneQueueLds = dev::lane::LaneLayout<true>::kBytes;
 // persistent workgroups for the BC1/BC3 encoder: 256 CUs x 4 resident workgroups
 constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
@@ -628,17 +629,18 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
     if (use_lane_encoder(s, (uint64_t)nblocks, path)) {
         const dim3 lblock(kThreads);
         if (d_queue && wants_lane_queue(s, w, h, path)) {
-            // persistent: the resident workgroups (3 per CU), fewer for small launches
-            const long long resident = 3LL * std::max(cu_count, 1);
-            const dim3 qgrid((unsigned)std::min<long long>(grid_for(nblocks, kThreads), resident));
+            // persistent: one workgroup of 12 waves per CU, fewer for small launches
+            const dim3 qblock(dev::lane::kQueueThreads);
+            const dim3 qgrid((unsigned)std::min<long long>(grid_for(nblocks, dev::lane::kQueueThreads),
+                                                           std::max(cu_count, 1)));
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
         if (bc3)                                                                                                    \
-            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, true>), qgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, \
-                               (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                     \
+            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, true>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h, \
+                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                 \
         else                                                                                                        \
-            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, false>), qgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw,\
-                               (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                     \
+            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, false>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h,\
+                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                 \
         break;
             switch (interp) {
                 QTX_LANE_CASE(QTX_INTERP_IDEAL)
