@@ -32,8 +32,9 @@ constexpr int kThreads = 256;       // workgroup of the static and batch lane ke
 constexpr int kQueueThreads = 768;  // workgroup of the queued kernel: one per CU, 3 waves per SIMD
 
 // Per-workgroup LDS of the lane kernels: this head, then (with palette tables,
-// PAL) the tables, then the cluster-fit prefix sums, [16][workgroup size]:
-// sums[k][t] = prefix sum k+1 of thread t's sorted texels (r | g << 16, b).
+// PAL) the tables, then the cluster-fit prefix sums, [17][workgroup size]:
+// sums[k][t] = sum of thread t's first k sorted texels (r | g << 16, b); row 0
+// stays zero, so a bin boundary at 0 needs no test.
 struct LaneHead {
     uint32_t match[2][256];  // this launch's interpolator, 5- and 6-bit
     float mid5[32], mid6[64];
@@ -50,12 +51,12 @@ struct LanePal {
 template <bool PAL> struct LaneLayout {
     static constexpr int kGroup = PAL ? kQueueThreads : kThreads;
     static constexpr size_t kSumsOffset = sizeof(LaneHead) + (PAL ? sizeof(LanePal) : 0);
-    static constexpr size_t kBytes = kSumsOffset + sizeof(uint2) * 16 * kGroup;
+    static constexpr size_t kBytes = kSumsOffset + sizeof(uint2) * 17 * kGroup;
 };
 static_assert(sizeof(LaneHead) % 16 == 0 && sizeof(LanePal) % 16 == 0, "LDS sections stay 16-byte aligned");
 __device__ __forceinline__ LaneHead &LL() { return *reinterpret_cast<LaneHead *>(qtx_lds_raw); }
 __device__ __forceinline__ LanePal &LP() { return *reinterpret_cast<LanePal *>(qtx_lds_raw + sizeof(LaneHead)); }
-// this thread's prefix sum k + 1
+// this thread's sum of its first k sorted texels
 template <bool PAL> __device__ __forceinline__ uint2 &lane_sum(uint32_t k) {
     return reinterpret_cast<uint2 *>(qtx_lds_raw + LaneLayout<PAL>::kSumsOffset)[k * LaneLayout<PAL>::kGroup + threadIdx.x];
 }
@@ -72,6 +73,7 @@ template <int INTERP, bool PAL> __device__ __forceinline__ void load_tables(cons
     for (int i = (int)threadIdx.x; i < 512; i += (int)blockDim.x) LL().match[i >> 8][i & 255] = T->match[INTERP][i >> 8][i & 255];
     if (threadIdx.x < 32) LL().mid5[threadIdx.x] = T->mid5[threadIdx.x];
     if (threadIdx.x < 64) LL().mid6[threadIdx.x] = T->mid6[threadIdx.x];
+    lane_sum<PAL>(0) = make_uint2(0, 0);
     if (PAL) {
         // computed by palette() itself, channel by channel (its channels are
         // independent), so a lookup equals the arithmetic exactly
@@ -652,7 +654,7 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     for (int i = 0; i < 16; i++) {
         sa += chn(sp[i], 0) | (chn(sp[i], 1) << 16);
         sb += chn(sp[i], 2);
-        lane_sum<PAL>((uint32_t)i) = make_uint2(sa, sb);
+        lane_sum<PAL>((uint32_t)i + 1) = make_uint2(sa, sb);
     }
     const float den = four ? 3.0f : 2.0f;
     const float s16_0 = (float)(sa & 0xFFFFu), s16_1 = (float)(sa >> 16), s16_2 = (float)sb;
@@ -694,9 +696,7 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
             // RefineEndpointsLS(result, sums, matrix, hash) (:827-850):
             // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10, exact integers
             const uint32_t l1 = lv & 0xFF, l2 = (lv >> 8) & 0xFF, l3 = (lv >> 16) & 0xFF;
-            const uint2 v1 = l1 ? lane_sum<PAL>(l1 - 1) : make_uint2(0, 0);
-            const uint2 v2 = l2 ? lane_sum<PAL>(l2 - 1) : make_uint2(0, 0);
-            const uint2 v3 = (four && l3) ? lane_sum<PAL>(l3 - 1) : make_uint2(0, 0);
+            const uint2 v1 = lane_sum<PAL>(l1), v2 = lane_sum<PAL>(l2), v3 = lane_sum<PAL>(four ? l3 : 0u);
             const uint32_t qa = v1.x + v2.x + v3.x, qb = v1.y + v2.y + v3.y;
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
