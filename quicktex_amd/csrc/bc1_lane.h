@@ -731,6 +731,7 @@ __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const B
     const int emode = three ? QTX_ERROR_FULL : four_error_mode;
     uint32_t prev = 0;
     int forbidden = -1;
+    bool improved = false;
     for (uint32_t i = 0; i < rounds; i++) {
         if ((int)(i & 31u) == forbidden) continue;
         const uint32_t v = i & 15u;
@@ -741,12 +742,19 @@ __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const B
         const uint32_t ep = (i & 16u) ? ((r.ep & 0xFFFF0000u) | ne) : ((r.ep & 0xFFFFu) | (ne << 16));
         const uint32_t err = trial_error<INTERP, PAL>(px, bk, M, emode, ep);
         if (err < r.err) {
-            r = Res{ep, 0u, err, M};
-            (void)find_selectors<INTERP, PAL>(px, bk, M, emode, ep, r.sel);
+            r.ep = ep;
+            r.err = err;
+            improved = true;
             forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
             prev = i;
         }
         if (i - prev > 32u) break;
+    }
+    // the selectors of the last improvement: a function of its endpoints and mode,
+    // computed once here rather than at every improvement
+    if (improved) {
+        r.mode = M;
+        (void)find_selectors<INTERP, PAL>(px, bk, M, emode, r.ep, r.sel);
     }
     return r;
 }
