@@ -127,7 +127,8 @@ def plain_leg(a, q, hip, np):
     achieved = [ops_launch[i] / (dur[:, i].mean() / 1000.0) / 1e12 for i in range(4)]
     achieved_mean = float(np.mean(achieved))
     roofline = {
-        'kernel': 'k_bc1_lane_queue (BC1 level 18, ThreeColorBlack, one block per lane, queued dealing)',
+        'kernel': 'k_bc1_lane_queue + k_bc1_search_drain (BC1 level 18, ThreeColorBlack, one block per lane, '
+                  'queued dealing; endpoint searches past round 40 parked and finished by the drain)',
         'bound': 'valu', 'unit': 'TOP/s',
         'achieved': round(achieved_mean, 3), 'peak': round(VALU_PEAK_TOPS, 2),
         'frac': round(achieved_mean / VALU_PEAK_TOPS, 4),

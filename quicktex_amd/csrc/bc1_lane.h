@@ -720,20 +720,32 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     return best;
 }
 
-// EndpointSearch (BC1Encoder.cpp:930-996), the reference's sequential loop: round i
-// moves the low (i & 16) or high endpoint by voxel i & 15, skips the reverse of the
-// last improving move, and stops 33 rounds after the last improvement.
+// Where an endpoint search stands: the next round, the last improving round, the
+// forbidden move (-1: none) and whether any round has improved.
+struct SearchState {
+    uint32_t next, prev;
+    int forbidden;
+    bool improved;
+};
+
+// Rounds st.next.. of EndpointSearch (BC1Encoder.cpp:930-996), the reference's
+// sequential loop: round i moves the low (i & 16) or high endpoint by voxel i & 15,
+// skips the reverse of the last improving move, and stops 33 rounds after the last
+// improvement.  Returns true when the search is over, false when it reached round
+// `stop` first (st then holds where to resume).  The selectors are left to
+// search_finish.
 template <int INTERP, bool PAL>
-__device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const Blk &bk, uint32_t rounds,
-                                               int four_error_mode, Res r) {
+__device__ __forceinline__ bool search_rounds(const uint32_t (&px)[16], const Blk &bk, uint32_t rounds,
+                                              int four_error_mode, Res &r, SearchState &st, uint32_t stop) {
     const bool three = r.mode == QTX_COLOR_THREE || r.mode == QTX_COLOR_THREE_BLACK;
     const int M = three ? r.mode : QTX_COLOR_FOUR;
     const int emode = three ? QTX_ERROR_FULL : four_error_mode;
-    uint32_t prev = 0;
-    int forbidden = -1;
-    bool improved = false;
-    for (uint32_t i = 0; i < rounds; i++) {
-        if ((int)(i & 31u) == forbidden) continue;
+    for (uint32_t i = st.next; i < rounds; i++) {
+        if (i >= stop) {
+            st.next = i;
+            return false;
+        }
+        if ((int)(i & 31u) == st.forbidden) continue;
         const uint32_t v = i & 15u;
         const uint32_t e = (i & 16u) ? (r.ep & 0xFFFFu) : (r.ep >> 16);
         const uint32_t ne = ep_make((uint32_t)clampi((int)ep_r(e) + vox_d(kVoxRp, v), 0, 31),
@@ -744,20 +756,90 @@ __device__ __forceinline__ Res endpoint_search(const uint32_t (&px)[16], const B
         if (err < r.err) {
             r.ep = ep;
             r.err = err;
-            improved = true;
-            forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
-            prev = i;
+            st.improved = true;
+            st.forbidden = (int)((kVoxOpp >> (4 * v)) & 15u) | (int)(i & 16u);
+            st.prev = i;
         }
-        if (i - prev > 32u) break;
+        if (i - st.prev > 32u) return true;
     }
-    // the selectors of the last improvement: a function of its endpoints and mode,
-    // computed once here rather than at every improvement
-    if (improved) {
-        r.mode = M;
-        (void)find_selectors<INTERP, PAL>(px, bk, M, emode, r.ep, r.sel);
-    }
-    return r;
+    return true;
 }
+
+// The selectors of a finished search: those of its last improvement, a function
+// of the endpoints and the mode (the mode of a search never changes: it is the
+// block's), computed once here rather than at every improvement.
+template <int INTERP, bool PAL>
+__device__ __forceinline__ void search_finish(const uint32_t (&px)[16], const Blk &bk, int four_error_mode, Res &r,
+                                              const SearchState &st) {
+    if (!st.improved) return;
+    const bool three = r.mode == QTX_COLOR_THREE || r.mode == QTX_COLOR_THREE_BLACK;
+    (void)find_selectors<INTERP, PAL>(px, bk, three ? r.mode : QTX_COLOR_FOUR, three ? QTX_ERROR_FULL : four_error_mode,
+                                      r.ep, r.sel);
+}
+
+// One speculative window of a search (EndpointSearch, BC1Encoder.cpp:930-996)
+// run by the G lanes of a group (lanes G*k .. G*k+G-1 of the wave, each holding
+// the block's texels and the same state): lane j evaluates the j-th round from
+// st.next that the sequential loop would not skip.  The first improving round is
+// taken, as the sequential loop takes it, and the next window starts after it;
+// with no improvement the window ends the search where the sequential loop would
+// (a round past the last, or one more than 32 rounds after the last improvement).
+// Returns true when the search is over.  Every lane of the group must call it.
+template <int INTERP, bool PAL, int G>
+__device__ __forceinline__ bool search_window(const uint32_t (&px)[16], const Blk &bk, uint32_t R,
+                                              int four_error_mode, Res &r, SearchState &st) {
+    static_assert(G >= 2 && G <= 16 && 64 % G == 0, "group of 2..16 lanes");
+    const uint32_t lane = threadIdx.x & 63u, j = lane % (uint32_t)G, gbase = lane - j;
+    constexpr uint32_t kAll = (1u << G) - 1u;
+    const bool three = r.mode == QTX_COLOR_THREE || r.mode == QTX_COLOR_THREE_BLACK;
+    const int M = three ? r.mode : QTX_COLOR_FOUR;
+    const int emode = three ? QTX_ERROR_FULL : four_error_mode;
+    const uint32_t i0 = st.next;
+    // the forbidden round at or after i0 (at most one in a window of <= 32 rounds)
+    const uint32_t f0 = st.forbidden >= 0 ? i0 + (((uint32_t)st.forbidden - (i0 & 31u)) & 31u) : 0xFFFFFFFFu;
+    uint32_t i = i0 + j;
+    if (i >= f0) i += 1;
+    uint32_t ip = i0 + j - 1u;  // the previous lane's round
+    if (j > 0 && ip >= f0) ip += 1;
+    // evaluated by the sequential loop unless an earlier round ended it
+    const bool valid = i < R && (j == 0 || ip <= st.prev + 32u);
+    uint32_t tep = r.ep, terr = 0xFFFFFFFFu;
+    if (valid) {
+        const uint32_t v = i & 15u;
+        const uint32_t e = (i & 16u) ? (r.ep & 0xFFFFu) : (r.ep >> 16);
+        const uint32_t ne = ep_make((uint32_t)clampi((int)ep_r(e) + vox_d(kVoxRp, v), 0, 31),
+                                    (uint32_t)clampi((int)ep_g(e) + vox_d(kVoxGp, v), 0, 63),
+                                    (uint32_t)clampi((int)ep_b(e) + vox_d(kVoxBp, v), 0, 31));
+        tep = (i & 16u) ? ((r.ep & 0xFFFF0000u) | ne) : ((r.ep & 0xFFFFu) | (ne << 16));
+        terr = trial_error<INTERP, PAL>(px, bk, M, emode, tep);
+    }
+    const uint32_t imp = (uint32_t)(__ballot(valid && terr < r.err) >> gbase) & kAll;
+    const uint32_t vmask = (uint32_t)(__ballot(valid) >> gbase) & kAll;
+    if (imp) {
+        const int src = (int)(gbase + (uint32_t)__builtin_ctz(imp));
+        const uint32_t ij = (uint32_t)__shfl((int)i, src), nep = (uint32_t)__shfl((int)tep, src);
+        const uint32_t nerr = (uint32_t)__shfl((int)terr, src);
+        r.ep = nep;
+        r.err = nerr;
+        st.improved = true;
+        st.forbidden = (int)((kVoxOpp >> (4 * (ij & 15u))) & 15u) | (int)(ij & 16u);
+        st.prev = ij;
+        st.next = ij + 1u;
+        return st.next >= R;
+    }
+    const uint32_t ilast = (uint32_t)__shfl((int)i, (int)(gbase + G - 1));
+    if (vmask != kAll || ilast > st.prev + 32u) return true;
+    st.next = ilast + 1u;
+    return st.next >= R;
+}
+
+// A search the caller may stop after its first rounds, to resume it elsewhere.
+struct SearchPark {
+    uint32_t stop;  // in: the round to stop at
+    bool parked;    // out: the search stopped with rounds left; r and st say where
+    Res r;
+    SearchState st;
+};
 
 // True when the lane encoder is the faster of the two for these settings: all
 // but the exhaustive cluster fit (level 19), whose 969 + 153 orderings per pass
@@ -766,9 +848,11 @@ __host__ __device__ constexpr bool lane_settings(int exhaustive) { return exhaus
 
 // BC1Encoder::EncodeBlock (BC1Encoder.cpp:261-342) for one block (texels RGBA8,
 // index x + 4y), any settings.
+// With pk, the endpoint search stops at round pk->stop: if it has rounds left then,
+// pk holds its state (pk->parked) and the returned block is not the block's.
 template <int INTERP, bool PAL>
 __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC1Settings &s,
-                                            const DeviceTables *__restrict__ T) {
+                                            const DeviceTables *__restrict__ T, SearchPark *pk = nullptr) {
     bool solid = true;
 #pragma unroll
     for (int i = 1; i < 16; i++) solid &= rgba[i] == rgba[0];  // alpha included (ColorBlock.h:80-86)
@@ -830,7 +914,16 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
     }
     if (result.err > 0 && s.search_rounds > 0) {
         touch(px);
-        result = endpoint_search<INTERP, PAL>(px, bk, s.search_rounds, s.error_mode, result);
+        SearchState st{0u, 0u, -1, false};
+        const bool over = search_rounds<INTERP, PAL>(px, bk, s.search_rounds, s.error_mode, result, st,
+                                                     pk ? pk->stop : s.search_rounds);
+        if (pk && !over) {
+            pk->parked = true;
+            pk->r = result;
+            pk->st = st;
+            return make_uint2(0u, 0u);
+        }
+        search_finish<INTERP, PAL>(px, bk, s.error_mode, result, st);
     }
     return write_block(result);
 }

@@ -71,6 +71,25 @@ QueueLease::QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted) : ctx_(ctx
     }
 }
 
+void QueueLease::reserve_park(size_t bytes) {
+    park = nullptr;
+    if (slot_ < 0 || bytes == 0) return;
+    void *&buf = ctx_->d_park[slot_];
+    size_t &cap = ctx_->park_cap[slot_];
+    if (cap < bytes) {  // the set's last launch has completed: its buffer is free
+        if (buf) (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
+        if (hipMalloc(&buf, bytes) != hipSuccess) {
+            buf = nullptr;
+            (void)hipGetLastError();
+            return;
+        }
+        cap = bytes;
+    }
+    park = buf;
+}
+
 hipError_t QueueLease::commit() {
     hipError_t e = hipSuccess;
     if (slot_ >= 0) e = hipEventRecord(ctx_->queue_done[slot_], stream_);
@@ -235,6 +254,8 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
         (void)hipEventDestroy(ev);
     }
     if (ctx->d_queues) (void)hipFree(ctx->d_queues);
+    for (void *p : ctx->d_park)
+        if (p) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QTX_OK;
@@ -268,8 +289,9 @@ qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
     QueueLease lease(ctx, (hipStream_t)stream, qtx::wants_lane_queue(s, width, height, ctx->bc1_path));
+    lease.reserve_park(qtx::bc1_park_bytes(s, width, height, ctx->bc1_path));
     QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false, ctx->bc1_path,
-                                   lease.words, ctx->cu_count, (hipStream_t)stream));
+                                   lease.words, lease.park, ctx->cu_count, (hipStream_t)stream));
     QTX_HIP(lease.commit());
     return QTX_OK;
 }
@@ -285,8 +307,9 @@ qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     if (s.color_mode != QTX_COLOR_FOUR) return fail(QTX_EINVAL, "BC3 colour blocks are always FourColor");
     if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
     QueueLease lease(ctx, (hipStream_t)stream, qtx::wants_lane_queue(s, width, height, ctx->bc1_path));
+    lease.reserve_park(qtx::bc1_park_bytes(s, width, height, ctx->bc1_path));
     QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, true, ctx->bc1_path,
-                                   lease.words, ctx->cu_count, (hipStream_t)stream));
+                                   lease.words, lease.park, ctx->cu_count, (hipStream_t)stream));
     QTX_HIP(lease.commit());
     return QTX_OK;
 }

@@ -54,11 +54,35 @@ constexpr double kLaneQueueMinOps = 10000.0;
 // Whether launch_bc1_encode would use queue words for this texture (so that the
 // caller need not lend a set otherwise).
 bool wants_lane_queue(const BC1Settings &s, int w, int h, int path);
+// Endpoint searches that go on past round kParkStop are parked by the queued
+// kernel -- their state written to a park buffer, their lane given the next block
+// -- and finished by a second kernel that runs one round per step on every lane
+// and hands a lane the next parked search as soon as its own is over, so that a
+// long search never holds 63 idle lanes.  Most searches end at round 33 or 34 (no
+// improvement); the park words count the parked searches and the drain's
+// finished waves.
+constexpr uint32_t kParkStop = 40;
+constexpr int kParkCountWord = 192;
+constexpr int kParkRetireWord = 193;
+constexpr size_t kParkRecordBytes = 32;
+// kParkStop, or QTX_PARK_STOP when set (0: no parking)
+inline uint32_t park_stop_env() {
+    static const uint32_t stop = [] {
+        const char *e = std::getenv("QTX_PARK_STOP");
+        return e ? (uint32_t)std::strtoul(e, nullptr, 10) : kParkStop;
+    }();
+    return stop;
+}
+// Bytes of the park buffer a queued launch of this texture needs (0: it parks no
+// search): one record per block at most.
+size_t bc1_park_bytes(const BC1Settings &s, int w, int h, int path);
 // d_queue: kLaneQueueWords zeroed words that no other launch uses until this one
 // has completed (the lane encoder then deals blocks from queues), or nullptr.
+// d_park: bc1_park_bytes bytes lent with d_queue (nullptr: searches run to the end
+// where they started).
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, int cu_count,
-                             hipStream_t stream);
+                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, void *d_park,
+                             int cu_count, hipStream_t stream);
 hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, int ch0, int ch1, bool bc5,
                              hipStream_t stream);
 // format: QTX_FMT_*; p0/p1: write_alpha for BC1, channels for BC4/BC5

@@ -218,11 +218,23 @@ __device__ __forceinline__ void retire_lane_queue(uint32_t *queue, uint32_t wave
     }
 }
 
+// A parked search (kParkRecordBytes): block index, endpoints, error, selectors;
+// next round, last improving round, mode | (forbidden move + 1) << 8 | improved << 16.
+__device__ __forceinline__ void park_search(uint4 *__restrict__ park, uint32_t slot, uint32_t gid,
+                                            const lane::SearchPark &pk) {
+    park[2 * slot] = make_uint4(gid, pk.r.ep, pk.r.err, pk.r.sel);
+    park[2 * slot + 1] = make_uint4(pk.st.next, pk.st.prev,
+                                    (uint32_t)pk.r.mode | ((uint32_t)(pk.st.forbidden + 1) << 8) |
+                                        ((uint32_t)pk.st.improved << 16),
+                                    0u);
+}
+
 template <int INTERP, bool BC3>
 __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
                                                         uint32_t nblocks, uint8_t *__restrict__ out,
                                                         const DeviceTables *__restrict__ T, BC1Settings s,
-                                                        uint32_t *__restrict__ queue) {
+                                                        uint32_t *__restrict__ queue, uint4 *__restrict__ park,
+                                                        uint32_t park_stop) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     lane::load_tables<INTERP, true>(T);
     // the wave's current range and the number of ranges it found used up live in
@@ -282,9 +294,122 @@ __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const
             asm volatile("" ::: "memory");
         }
         if (__ballot(gid != kNone) == 0) break;
-        if (gid != kNone) store_lane_block(out, gid, px, lane::encode_bc1<INTERP, true>(px, s, T), BC3);
+        lane::SearchPark pk;
+        pk.stop = park_stop;
+        pk.parked = false;
+        uint2 colour = make_uint2(0u, 0u);
+        if (gid != kNone) colour = lane::encode_bc1<INTERP, true>(px, s, T, &pk);
+        // every lane of the wave is active again here.  Searches with rounds left go
+        // to the park buffer: one slot claim per wave; a block parks at most once,
+        // so the slots never exceed nblocks.
+        const uint64_t parked = __ballot(pk.parked);
+        if (parked != 0) {
+            uint32_t base = 0;
+            if ((threadIdx.x & 63) == 0) base = atomicAdd(&queue[kParkCountWord], (uint32_t)__popcll(parked));
+            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+            if (pk.parked) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(parked >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)parked, 0u));
+                park_search(park, base + rank, gid, pk);
+            }
+        }
+        if (gid != kNone && !pk.parked) store_lane_block(out, gid, px, colour, BC3);
     }
     retire_lane_queue(queue, gridDim.x * (blockDim.x / 64));
+}
+
+// The drain gives each parked search a group of kDrainGroup lanes, which evaluate
+// that many rounds of it at once (a speculative window), and deals the searches
+// to the waves in equal contiguous shares (no atomics: one counter taking every
+// claim would serialise them), at least kDrainPerGroup per group: enough waves to
+// spread them over the CUs, few enough that a group whose search ends finds
+// another to take.
+constexpr int kDrainGroup = 8;
+constexpr uint32_t kDrainGroups = 64 / kDrainGroup;  // groups per wave
+constexpr uint32_t kDrainPerGroup = 2;
+// A wave hands out new searches once this many of its groups are free.
+constexpr uint32_t kDrainRefill = 2;
+
+// The second half of the parked endpoint searches (k_bc1_lane_queue): each step
+// runs one window of rounds of every busy group's search; finished searches get
+// their selectors and are stored, and their groups take the next searches of the
+// wave's share (in batches of kDrainRefill groups).  A wave stops when its share
+// is used up and its groups are free: no wave waits for another.  The last wave
+// to finish returns the park words to zero.
+template <int INTERP, bool BC3>
+__global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_search_drain(
+    const uint8_t *__restrict__ rgba, int w, int h, int bw, uint8_t *__restrict__ out,
+    const DeviceTables *__restrict__ T, BC1Settings s, const uint4 *__restrict__ park, uint32_t *__restrict__ queue) {
+    lane::load_tables<INTERP, true>(T);
+    __syncthreads();
+    const uint32_t total = queue[kParkCountWord];  // final: counted by the previous launch
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)));
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    const uint32_t j = threadIdx.x % kDrainGroup;                 // lane in its group
+    const uint32_t g = (threadIdx.x & 63u) / (uint32_t)kDrainGroup;  // group in the wave
+    // this wave's share of the parked searches: [cursor, end)
+    const uint32_t used = min(waves, (total + kDrainGroups * kDrainPerGroup - 1) / (kDrainGroups * kDrainPerGroup));
+    const uint32_t share = used ? (total + used - 1) / used : 0u;
+    uint32_t cursor = min(wave * share, total);
+    const uint32_t end = min(cursor + share, total);
+    if (cursor < end) {
+        // the state of the group's search, the same in each of its lanes
+        bool has = false, done = false, open = true;
+        uint32_t gid = 0, px[16];
+        lane::Blk bk{0u, 0u};
+        Res r{0u, 0u, 0u, 0};
+        lane::SearchState st{0u, 0u, -1, false};
+        for (;;) {
+            const uint32_t busy = (uint32_t)__popcll(__ballot(has && !done)) / (uint32_t)kDrainGroup;
+            if (kDrainGroups - busy >= kDrainRefill || busy == 0) {
+                if (done) {  // finish: selectors, then the block (BC3: its alpha from the source)
+                    lane::search_finish<INTERP, true>(px, bk, s.error_mode, r, st);
+                    const uint2 colour = write_block(r);
+                    if (j == 0) {
+                        if (BC3) load_block16(rgba, w, h, bw, (int)gid, px);
+                        store_lane_block(out, gid, px, colour, BC3);
+                    }
+                    has = false;
+                    done = false;
+                }
+                if (open) {
+                    const uint64_t want = __ballot(!has && j == 0);  // the free groups' first lanes
+                    const uint32_t nwant = (uint32_t)__popcll(want);
+                    const uint32_t base = cursor;
+                    const uint32_t got = min(nwant, end - base);
+                    cursor += got;
+                    if (cursor == end) open = false;
+                    if (!has) {
+                        const uint32_t rank = (uint32_t)__popcll(want & ((1ull << (g * kDrainGroup)) - 1ull));
+                        if (rank < got) {
+                            const uint4 a = park[2 * (base + rank)], b = park[2 * (base + rank) + 1];
+                            gid = a.x;
+                            r = Res{a.y, a.w, a.z, (int)(b.z & 0xFFu)};
+                            st = lane::SearchState{b.x, b.y, (int)((b.z >> 8) & 0xFFu) - 1, ((b.z >> 16) & 1u) != 0};
+                            load_block16(rgba, w, h, bw, (int)gid, px);
+                            bk.pp = 0u;
+#pragma unroll
+                            for (int i = 0; i < 16; i++) {
+                                px[i] &= 0x00FFFFFFu;
+                                bk.pp += dot4(px[i], px[i]);
+                            }
+                            has = true;
+                        }
+                    }
+                }
+                if (__ballot(has) == 0) break;
+            }
+            if (has && !done)
+                done = lane::search_window<INTERP, true, kDrainGroup>(px, bk, s.search_rounds, s.error_mode, r, st);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[kParkRetireWord], 1u) == gridDim.x * (blockDim.x / 64) - 1) {
+            atomicExch(&queue[kParkCountWord], 0u);
+            atomicExch(&queue[kParkRetireWord], 0u);
+        }
+    }
 }
 
 // BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
@@ -587,6 +712,8 @@ constexpr int kThreads = 256;
 constexpr size_t kBc1Lds = sizeof(dev::Lds);
 constexpr size_t kLaneLds = dev::lane::LaneLayout<false>::kBytes;
 constexpr size_t kLaneQueueLds = dev::lane::LaneLayout<true>::kBytes;
+// the drain needs the palette tables and the zero row of the prefix sums only
+constexpr size_t kDrainLds = dev::lane::LaneLayout<true>::kSumsOffset + sizeof(uint2) * dev::lane::kQueueThreads;
 // persistent workgroups for the BC1/BC3 encoder: 256 CUs x 4 resident workgroups
 constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
@@ -621,9 +748,15 @@ bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
     return use_lane_encoder(s, nblocks, path) && lane_queue_env() && bc1_block_ops(s) >= kLaneQueueMinOps;
 }
 
+size_t bc1_park_bytes(const BC1Settings &s, int w, int h, int path) {
+    const uint32_t stop = park_stop_env();
+    if (!wants_lane_queue(s, w, h, path) || stop == 0 || s.search_rounds <= stop) return 0;
+    return kParkRecordBytes * (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4);
+}
+
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
-                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, int cu_count,
-                             hipStream_t stream) {
+                             const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, void *d_park,
+                             int cu_count, hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
     if (use_lane_encoder(s, (uint64_t)nblocks, path)) {
@@ -633,14 +766,26 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             const dim3 qblock(dev::lane::kQueueThreads);
             const dim3 qgrid((unsigned)std::min<long long>(grid_for(nblocks, dev::lane::kQueueThreads),
                                                            std::max(cu_count, 1)));
+            // parking (and the drain launch after the queued kernel) only with a park buffer
+            const bool drain = d_park && bc1_park_bytes(s, w, h, path) > 0;
+            const uint32_t stop = drain ? park_stop_env() : 0xFFFFFFFFu;
+            uint4 *const park = static_cast<uint4 *>(d_park);
+            const dim3 dgrid((unsigned)std::max(cu_count, 1));
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
-        if (bc3)                                                                                                    \
+        if (bc3) {                                                                                                  \
             hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, true>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h, \
-                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                 \
-        else                                                                                                        \
+                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop);                     \
+            if (drain)                                                                                              \
+                hipLaunchKernelGGL((dev::k_bc1_search_drain<I, true>), dgrid, qblock, kDrainLds, stream, d_rgba, w, \
+                                   h, bw, d_out, d_tables, s, park, d_queue);                                       \
+        } else {                                                                                                    \
             hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, false>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h,\
-                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue);                                 \
+                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop);                     \
+            if (drain)                                                                                              \
+                hipLaunchKernelGGL((dev::k_bc1_search_drain<I, false>), dgrid, qblock, kDrainLds, stream, d_rgba, w,\
+                                   h, bw, d_out, d_tables, s, park, d_queue);                                       \
+        }                                                                                                           \
         break;
             switch (interp) {
                 QTX_LANE_CASE(QTX_INTERP_IDEAL)

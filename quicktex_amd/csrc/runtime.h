@@ -31,6 +31,10 @@ struct qtx_ctx {
     static constexpr int kQueueSlots = 64;
     uint32_t *d_queues = nullptr;
     hipEvent_t queue_done[kQueueSlots] = {};  // created on a slot's first use
+    // Park buffers of the queued kernel's endpoint searches, one per set, grown on
+    // demand (qtx::bc1_park_bytes).
+    void *d_park[kQueueSlots] = {};
+    size_t park_cap[kQueueSlots] = {};
     std::mutex queue_mu;
 };
 
@@ -59,6 +63,11 @@ class QueueLease {
     QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted);
     ~QueueLease();
     uint32_t *words = nullptr;
+    // The set's park buffer, at least `bytes` long (grown if need be), or nullptr
+    // when there is no set, bytes is 0 or the allocation fails (the launch then
+    // parks nothing).
+    void *park = nullptr;
+    void reserve_park(size_t bytes);
     hipError_t commit();
 
   private:

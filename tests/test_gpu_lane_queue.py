@@ -6,7 +6,10 @@ against the CPU checker: relaunches on one stream (the words must be back at
 zero, also when a smaller texture follows a larger one), many launches in
 flight on many streams at once (each holds its own set; past 64 the queue-less
 kernel takes over), and the dealing's edge cases: all blocks solid, none solid,
-one block, ragged sizes.
+one block, ragged sizes.  Endpoint searches that run past the park stop are
+parked by the queued kernel and finished by the drain kernel: checked at level
+18 for every interpolator, BC1 and BC3, relaunched on one stream (the park words
+return to zero) and in flight on several streams (one park buffer per set).
 """
 import numpy as np
 import pytest
@@ -121,3 +124,48 @@ def test_dealing_edges(lane, checker, kind, bc3):
     else:
         got, want = lane.bc1_encode(img, p), checker.bc1_encode(img, p)
     assert (got == want).all()
+
+
+# Endpoint searches longer than the park stop (round 40) are parked by the queued
+# kernel and finished by the drain kernel; level 18 searches up to 256 rounds.
+PARK_LEVEL = 18
+
+
+@pytest.mark.parametrize('interp', [0, 1, 2, 3])
+@pytest.mark.parametrize('bc3', [False, True])
+def test_parked_searches(lane, checker, interp, bc3):
+    img = synth.mixture_rgba(160, 96, seed=40 + interp, near_black=0.1)
+    if bc3:
+        img[..., 3] = synth.alpha_channel(160, 96, seed=50 + interp)
+    p = lane.params(PARK_LEVEL, 4 if bc3 else 0x13, interp)
+    if bc3:
+        got, want = lane.bc3_encode(img, p), checker.bc3_encode(img, p)
+    else:
+        got, want = lane.bc1_encode(img, p), checker.bc1_encode(img, p)
+    assert (got == want).all()
+
+
+def test_parked_relaunch_and_streams(lane, checker):
+    """The park words return to zero after each launch (a smaller texture after a
+    larger one on one stream), and launches in flight on several streams each park
+    into their own set's buffer."""
+    from quicktex_amd import hip
+    q = lane
+    big = synth.mixture_rgba(256, 192, seed=61, near_black=0.1)
+    small = synth.mixture_rgba(72, 36, seed=62, near_black=0.1)
+    p = q.params(PARK_LEVEL, 0x13, 2)
+    want_big, want_small = checker.bc1_encode(big, p), checker.bc1_encode(small, p)
+    s = hip.Stream()
+    for img, want in ((big, want_big), (small, want_small), (big, want_big)):
+        d_in, d_out = _encode_device(q, hip, img, p, s)
+        s.synchronize()
+        assert (d_out.download().reshape(-1) == want).all(), img.shape
+    s.destroy()
+    streams = [hip.Stream() for _ in range(6)]
+    held = [_encode_device(q, hip, big if i % 2 == 0 else small, p, st) for i, st in enumerate(streams)]
+    hip.synchronize()
+    for i, (d_in, d_out) in enumerate(held):
+        want = want_big if i % 2 == 0 else want_small
+        assert (d_out.download().reshape(-1) == want).all(), 'stream %d' % i
+    for st in streams:
+        st.destroy()
