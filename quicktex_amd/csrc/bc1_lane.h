@@ -114,6 +114,25 @@ template <int INTERP, bool PAL> __device__ __forceinline__ void lane_palette(uin
     c[3] = __builtin_amdgcn_perm(b, rg23, 0x0C070302u);
 }
 
+// The same palette from the endpoints' components e (low r g b, high r g b), which
+// index the tables without unpacking a pair.
+template <int INTERP, bool PAL>
+__device__ __forceinline__ void lane_palette_parts(const uint32_t (&e)[6], bool three, uint32_t c[4]) {
+    if (!PAL) {
+        palette<INTERP>(rgb(e[0], e[1], e[2]), rgb(e[3], e[4], e[5]), three, c);
+        return;
+    }
+    const uint32_t *t5 = three ? LP().three5 : LP().four5;
+    const uint32_t *t6 = three ? LP().three6 : LP().four6;
+    const uint32_t r = t5[e[0] | (e[3] << 5)], g = t6[e[1] | (e[4] << 6)], b = t5[e[2] | (e[5] << 5)];
+    const uint32_t rg01 = __builtin_amdgcn_perm(g, r, 0x05010400u);
+    const uint32_t rg23 = __builtin_amdgcn_perm(g, r, 0x07030602u);
+    c[0] = __builtin_amdgcn_perm(b, rg01, 0x0C040100u);
+    c[1] = __builtin_amdgcn_perm(b, rg01, 0x0C050302u);
+    c[2] = __builtin_amdgcn_perm(b, rg23, 0x0C060100u);
+    c[3] = __builtin_amdgcn_perm(b, rg23, 0x0C070302u);
+}
+
 // Color::PreciseRound565 (quicktex/Color.cpp:163-183)
 __device__ __forceinline__ uint32_t prnd565(float r, float g, float b) {
     int ir = clampi(cvt_trunc(r * 31.0f), 0, 31);
@@ -131,15 +150,22 @@ __device__ __forceinline__ uint32_t prnd565(float r, float g, float b) {
 // determinant, times 3/255 or 2/255, so |coefficient| < 2, and the texel sums are
 // <= 3 x 16 x 255, so |component| < 2^17.  The truncating conversion then needs
 // no out-of-range case, and the six rounding-point reads are issued together.
-__device__ __forceinline__ uint32_t prnd565_ls(float lr, float lg, float lb, float hr, float hg, float hb) {
+// e gets the 5:6:5 components, low r g b then high r g b.
+__device__ __forceinline__ void prnd565_ls(float lr, float lg, float lb, float hr, float hg, float hb, uint32_t (&e)[6]) {
     const int ilr = clampi((int)(lr * 31.0f), 0, 31), ilg = clampi((int)(lg * 63.0f), 0, 63);
     const int ilb = clampi((int)(lb * 31.0f), 0, 31), ihr = clampi((int)(hr * 31.0f), 0, 31);
     const int ihg = clampi((int)(hg * 63.0f), 0, 63), ihb = clampi((int)(hb * 31.0f), 0, 31);
     const float mlr = LL().mid5[ilr], mlg = LL().mid6[ilg], mlb = LL().mid5[ilb];
     const float mhr = LL().mid5[ihr], mhg = LL().mid6[ihg], mhb = LL().mid5[ihb];
-    const uint32_t lo = ep_make((uint32_t)(ilr + (lr > mlr)), (uint32_t)(ilg + (lg > mlg)), (uint32_t)(ilb + (lb > mlb)));
-    const uint32_t hi = ep_make((uint32_t)(ihr + (hr > mhr)), (uint32_t)(ihg + (hg > mhg)), (uint32_t)(ihb + (hb > mhb)));
-    return lo | (hi << 16);
+    e[0] = (uint32_t)(ilr + (lr > mlr));
+    e[1] = (uint32_t)(ilg + (lg > mlg));
+    e[2] = (uint32_t)(ilb + (lb > mlb));
+    e[3] = (uint32_t)(ihr + (hr > mhr));
+    e[4] = (uint32_t)(ihg + (hg > mhg));
+    e[5] = (uint32_t)(ihb + (hb > mhb));
+}
+__device__ __forceinline__ uint32_t ep_pair(const uint32_t (&e)[6]) {
+    return ep_make(e[0], e[1], e[2]) | (ep_make(e[3], e[4], e[5]) << 16);
 }
 
 // -v, opaque to the optimiser: 2 p.c - c.c is then written (p.c << 1) + (-c.c), one
@@ -384,10 +410,8 @@ struct Blk {
 // error (0 for ErrorMode::None, which computes none), sel gets the selectors
 // (internal order).  |p - c|^2 = p.p - (2 p.c - c.c), with v_dot4_u32_u8.
 template <int INTERP, bool PAL>
-__device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
-                                                   uint32_t ep, uint32_t &sel) {
-    uint32_t c[4];
-    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
+__device__ __forceinline__ uint32_t find_selectors_c(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
+                                                     const uint32_t (&c)[4], uint32_t &sel) {
     const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
     const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
     int tot = 0;
@@ -463,10 +487,8 @@ __device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], con
 // The error find_selectors returns, without the selectors: what cluster-fit and
 // endpoint-search trials need (the winner's selectors are computed once).
 template <int INTERP, bool PAL>
-__device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
-                                                uint32_t ep) {
-    uint32_t c[4];
-    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
+__device__ __forceinline__ uint32_t trial_error_c(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
+                                                  const uint32_t (&c)[4]) {
     int tot = 0;
     if (error_mode == QTX_ERROR_FULL) {
         const uint32_t n0 = neg_opaque(dot4(c[0], c[0])), n1 = neg_opaque(dot4(c[1], c[1]));
@@ -511,7 +533,23 @@ __device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const 
         return bk.pp - (uint32_t)tot;
     }
     uint32_t sel;
-    return find_selectors<INTERP, PAL>(px, bk, M, error_mode, ep, sel);
+    return find_selectors_c<INTERP, PAL>(px, bk, M, error_mode, c, sel);
+}
+
+// find_selectors and trial_error for endpoints ep (low | high << 16)
+template <int INTERP, bool PAL>
+__device__ __forceinline__ uint32_t find_selectors(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
+                                                   uint32_t ep, uint32_t &sel) {
+    uint32_t c[4];
+    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
+    return find_selectors_c<INTERP, PAL>(px, bk, M, error_mode, c, sel);
+}
+template <int INTERP, bool PAL>
+__device__ __forceinline__ uint32_t trial_error(const uint32_t (&px)[16], const Blk &bk, int M, int error_mode,
+                                                uint32_t ep) {
+    uint32_t c[4];
+    lane_palette<INTERP, PAL>(ep, (M & 0xF) == 3, c);
+    return trial_error_c<INTERP, PAL>(px, bk, M, error_mode, c);
 }
 
 // FindEndpointsSingleColor(result, pixels, color, is_3color) (BC1Encoder.cpp:458-471):
@@ -558,8 +596,10 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uin
     const float a0 = (m3 * 1.0f) * scale, a1 = (m1 * -1.0f) * scale, a2 = (m1 * -1.0f) * scale, a3 = (m0 * 1.0f) * scale;
     const float q0 = (float)(qa & 0xFFFFu), q1 = (float)(qa >> 16), q2 = (float)qb;
     const float r0 = (float)(m.s0 * 3) - q0, r1 = (float)(m.s1 * 3) - q1, r2 = (float)(m.s2 * 3) - q2;
-    return prnd565_ls((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2), (a2 * q0) + (a3 * r0),
-                      (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2));
+    uint32_t e[6];
+    prnd565_ls((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2), (a2 * q0) + (a3 * r0),
+               (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2), e);
+    return ep_pair(e);
 }
 
 // RefineBlockLS<M> (BC1Encoder.cpp:852-875)
@@ -680,6 +720,14 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
     // are computed once at the end
     uint32_t b_err = orig.err, b_ep = orig.ep;
     bool b_single = false, improved = false;
+    // The trial of a single-bin ordering (FindEndpointsSingleColor, :458-471) depends
+    // only on the block and orig.  With many orderings (the queued kernel, PAL) it
+    // is computed once here and selected where such an ordering comes up, after the
+    // least-squares trial that every lane runs (its factors are zero there; its
+    // result is dropped), so that no lane branches off; with few, where a branch
+    // is cheaper than the up-front trial, it is computed where it comes up.
+    Res single{0u, 0u, 0u, 0};
+    if (PAL) single = single_color_px<INTERP, PAL>(px, bk, orig.ep, m.avg, !four);
     for (uint32_t q = 0; q < q_total; q++) {
         const uint32_t lv = lv_next;
         const float4 fm = fm_next;
@@ -687,28 +735,40 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
         fm_next = factors[h_next];
         const uint32_t q2 = min(q + 2u, q_last);
         h_next = exhaustive ? q2 : best_row[q2];
-        uint32_t t_err, t_ep;
-        if (lv >> 24) {
-            const Res t = single_color_px<INTERP, PAL>(px, bk, orig.ep, m.avg, !four);
-            t_err = t.err;
-            t_ep = t.ep;
-        } else {
-            // RefineEndpointsLS(result, sums, matrix, hash) (:827-850):
-            // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10, exact integers
+        // RefineEndpointsLS(result, sums, matrix, hash) (:827-850):
+        // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10, exact integers
+        const auto ls_trial = [&](uint32_t &ep) {
             const uint32_t l1 = lv & 0xFF, l2 = (lv >> 8) & 0xFF, l3 = (lv >> 16) & 0xFF;
             const uint2 v1 = lane_sum<PAL>(l1), v2 = lane_sum<PAL>(l2), v3 = lane_sum<PAL>(four ? l3 : 0u);
             const uint32_t qa = v1.x + v2.x + v3.x, qb = v1.y + v2.y + v3.y;
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
-            t_ep = prnd565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
-                              (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
-                              (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2));
-            t_err = trial_error<INTERP, PAL>(px, bk, M, error_mode, t_ep);
+            uint32_t e[6], c[4];
+            prnd565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
+                       (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
+                       (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2), e);
+            lane_palette_parts<INTERP, PAL>(e, !four, c);
+            ep = ep_pair(e);
+            return trial_error_c<INTERP, PAL>(px, bk, M, error_mode, c);
+        };
+        const bool is_single = (lv >> 24) != 0;
+        uint32_t t_err, t_ep;
+        if (PAL) {
+            uint32_t ls_ep;
+            const uint32_t ls_err = ls_trial(ls_ep);
+            t_err = is_single ? single.err : ls_err;
+            t_ep = is_single ? single.ep : ls_ep;
+        } else if (is_single) {
+            const Res t = single_color_px<INTERP, PAL>(px, bk, orig.ep, m.avg, !four);
+            t_err = t.err;
+            t_ep = t.ep;
+        } else {
+            t_err = ls_trial(t_ep);
         }
         if (t_err < b_err) {
             b_err = t_err;
             b_ep = t_ep;
-            b_single = (lv >> 24) != 0;
+            b_single = is_single;
             improved = true;
         }
         if (t_err == 0) break;
