@@ -4,16 +4,18 @@
 Headline (value): BC1 level-18 encode of a 4096x4096 RGBA8 texture -- block
 mixture of 22.5% solid, 22.5% two-colour, 45% noisy gradient and 10% near-black
 blocks (quicktex_amd/synth.py) -- in ThreeColorBlack mode, once per
-interpolator (Ideal, IdealRound, Nvidia, AMD), one plain launch per texture.
-One step = the four encodes, inputs already resident in HBM.  value =
+interpolator (Ideal, IdealRound, Nvidia, AMD), one plain encode per texture
+(the queued lane kernel, then the drain kernel for the endpoint searches it
+parked).  One step = the four encodes, inputs already resident in HBM.  value =
 Mpixel/s over the timed steps.
 
-roofline: that kernel against the VALU peak (256 CU x 4 SIMD x 16 lanes x
+roofline: the encode against the VALU peak (256 CU x 4 SIMD x 16 lanes x
 2.4 GHz = 39.3 T lane-ops/s; the encoder is integer/FP32 ALU work, no HBM or
 MFMA bound); achieved = algorithmic ops (oracle accounting on a sample, scaled
-to the launch) over the kernel's mean duration from HIP events on its stream.
-traffic = HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of the
-committed profile, scaled by a plain copy kernel (see --copy-probe).
+to the texture) over the encode's mean duration (both kernels) from HIP events
+on its stream.  traffic = HBM bytes per encode from the FETCH_SIZE / WRITE_SIZE
+counters of the committed profile, scaled by a plain copy kernel (see
+--copy-probe).
 cpu_baseline: the reference's own C++ (oracle/_ref, OpenMP over block rows like
 the reference) timed on this host on the first --cpu-rows rows of the same
 texture; the GPU output of those rows is checked bit-exact against it.
@@ -54,10 +56,10 @@ METRIC = 'BC1 L18 encode Mpixel/s bit-exact, batched/served vs plain; p50/p99 la
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # non-packed 32-bit lane-ops per second, T
 W = H = 4096
 INTERPS = ('Ideal', 'IdealRound', 'Nvidia', 'AMD')
-# HBM bytes per BC1 encode launch on this workload (k_bc1_lane_queue), from the
-# committed counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000 (copy-kernel calibration), mean over the four interpolators.
-TRAFFIC_BYTES = (286.39e6 + 286.83e6 + 284.13e6 + 286.90e6) / 4
+# HBM bytes per BC1 encode on this workload (k_bc1_lane_queue + k_bc1_search_drain),
+# from the committed counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2 +
+# WRITE_SIZE x 1 (copy-kernel calibration), mean over the four interpolators.
+TRAFFIC_BYTES = (433.48e6 + 432.73e6 + 453.84e6 + 440.27e6) / 4
 
 
 def log(*a):
@@ -128,14 +130,14 @@ def plain_leg(a, q, hip, np):
     achieved_mean = float(np.mean(achieved))
     roofline = {
         'kernel': 'k_bc1_lane_queue + k_bc1_search_drain (BC1 level 18, ThreeColorBlack, one block per lane, '
-                  'queued dealing; endpoint searches past round 40 parked and finished by the drain)',
+                  'queued dealing; endpoint searches past round 34 parked and finished by the drain)',
         'bound': 'valu', 'unit': 'TOP/s',
         'achieved': round(achieved_mean, 3), 'peak': round(VALU_PEAK_TOPS, 2),
         'frac': round(achieved_mean / VALU_PEAK_TOPS, 4),
         'algorithmic_ops_per_block': round(float(np.mean(ops_block)), 1),
         'kernel_ms_mean': round(float(dur.mean()), 3),
         'traffic': round(TRAFFIC_BYTES),
-        'traffic_unit': 'HBM bytes per launch (counter profile)',
+        'traffic_unit': 'HBM bytes per encode, both kernels (counter profile)',
         'algorithmic_bytes': W * H * 4 + nb * 8,
     }
 
