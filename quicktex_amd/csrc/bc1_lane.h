@@ -901,6 +901,25 @@ struct SearchPark {
     SearchState st;
 };
 
+// The settings every preset of levels 13-18 shares (BC1Encoder.cpp:74-249): Full
+// error, PCA endpoints, two least-squares and two endpoint passes, no exhaustive
+// cluster fit.  fixed_settings<true> returns s with these as constants.
+__host__ __device__ inline bool hq_settings(const BC1Settings &s) {
+    return s.error_mode == QTX_ERROR_FULL && s.endpoint_mode == QTX_EP_PCA && s.ls_passes == 2 &&
+           s.ep_passes == 2 && s.exhaustive == 0;
+}
+template <bool HQ> __device__ __forceinline__ BC1Settings fixed_settings(const BC1Settings &s) {
+    BC1Settings t = s;
+    if (HQ) {
+        t.error_mode = QTX_ERROR_FULL;
+        t.endpoint_mode = QTX_EP_PCA;
+        t.ls_passes = 2;
+        t.ep_passes = 2;
+        t.exhaustive = 0;
+    }
+    return t;
+}
+
 // True when the lane encoder is the faster of the two for these settings: all
 // but the exhaustive cluster fit (level 19), whose 969 + 153 orderings per pass
 // the 16-lane encoder deals to 16 lanes.

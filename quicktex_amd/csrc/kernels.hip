@@ -229,13 +229,16 @@ __device__ __forceinline__ void park_search(uint4 *__restrict__ park, uint32_t s
                                     0u);
 }
 
-template <int INTERP, bool BC3>
+// HQ: the settings of levels 13-18 as constants (lane::hq_settings), so that the
+// encoder carries no branch on them and no registers for them.
+template <int INTERP, bool BC3, bool HQ>
 __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
                                                         uint32_t nblocks, uint8_t *__restrict__ out,
-                                                        const DeviceTables *__restrict__ T, BC1Settings s,
+                                                        const DeviceTables *__restrict__ T, BC1Settings s_in,
                                                         uint32_t *__restrict__ queue, uint4 *__restrict__ park,
                                                         uint32_t park_stop) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
+    const BC1Settings s = lane::fixed_settings<HQ>(s_in);
     lane::load_tables<INTERP, true>(T);
     // the wave's current range and the number of ranges it found used up live in
     // LDS between refills, so that they hold no registers through the searches
@@ -771,17 +774,21 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             const uint32_t stop = drain ? park_stop_env() : 0xFFFFFFFFu;
             uint4 *const park = static_cast<uint4 *>(d_park);
             const dim3 dgrid((unsigned)std::max(cu_count, 1));
+            const bool hq = dev::lane::hq_settings(s);
+#define QTX_QUEUE_LAUNCH(I, B, H)                                                                                   \
+    hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, B, H>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h, bw,    \
+                       (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop)
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
         if (bc3) {                                                                                                  \
-            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, true>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h, \
-                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop);                     \
+            if (hq) QTX_QUEUE_LAUNCH(I, true, true);                                                                \
+            else QTX_QUEUE_LAUNCH(I, true, false);                                                                  \
             if (drain)                                                                                              \
                 hipLaunchKernelGGL((dev::k_bc1_search_drain<I, true>), dgrid, qblock, kDrainLds, stream, d_rgba, w, \
                                    h, bw, d_out, d_tables, s, park, d_queue);                                       \
         } else {                                                                                                    \
-            hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, false>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h,\
-                               bw, (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop);                     \
+            if (hq) QTX_QUEUE_LAUNCH(I, false, true);                                                               \
+            else QTX_QUEUE_LAUNCH(I, false, false);                                                                 \
             if (drain)                                                                                              \
                 hipLaunchKernelGGL((dev::k_bc1_search_drain<I, false>), dgrid, qblock, kDrainLds, stream, d_rgba, w,\
                                    h, bw, d_out, d_tables, s, park, d_queue);                                       \
@@ -795,6 +802,7 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
                 default: return hipErrorInvalidValue;
             }
 #undef QTX_LANE_CASE
+#undef QTX_QUEUE_LAUNCH
             return hipGetLastError();
         }
         const dim3 lgrid(grid_for(nblocks, kThreads));
