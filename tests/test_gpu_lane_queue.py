@@ -126,7 +126,7 @@ def test_dealing_edges(lane, checker, kind, bc3):
     assert (got == want).all()
 
 
-# Endpoint searches longer than the park stop (round 40) are parked by the queued
+# Endpoint searches longer than the park stop (round 34) are parked by the queued
 # kernel and finished by the drain kernel; level 18 searches up to 256 rounds.
 PARK_LEVEL = 18
 
