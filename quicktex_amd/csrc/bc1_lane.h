@@ -31,10 +31,12 @@ namespace lane {
 constexpr int kThreads = 256;       // workgroup of the static and batch lane kernels
 constexpr int kQueueThreads = 768;  // workgroup of the queued kernel: one per CU, 3 waves per SIMD
 
-// Per-workgroup LDS of the lane kernels: this head, then (with palette tables,
-// PAL) the tables, then the cluster-fit prefix sums, [17][workgroup size]:
-// sums[k][t] = sum of thread t's first k sorted texels (r | g << 16, b); row 0
-// stays zero, so a bin boundary at 0 needs no test.
+// Per-workgroup LDS of the lane kernels, allocated statically (addresses known at
+// compile time fold into the LDS instructions' offsets): a head, then (with
+// palette tables, PAL: the queued kernel and the drain) the tables, then the
+// cluster-fit prefix sums, [17][workgroup size]: sums[k][t] = sum of thread t's
+// first k sorted texels (r | g << 16, b); row 0 stays zero, so a bin boundary at
+// 0 needs no test.
 struct LaneHead {
     uint32_t match[2][256];  // this launch's interpolator, 5- and 6-bit
     float mid5[32], mid6[64];
@@ -48,17 +50,27 @@ constexpr int kPal5 = 32 * 32, kPal6 = 64 * 64;
 struct LanePal {
     uint32_t four5[kPal5], four6[kPal6], three5[kPal5], three6[kPal6];
 };
-template <bool PAL> struct LaneLayout {
-    static constexpr int kGroup = PAL ? kQueueThreads : kThreads;
-    static constexpr size_t kSumsOffset = sizeof(LaneHead) + (PAL ? sizeof(LanePal) : 0);
-    static constexpr size_t kBytes = kSumsOffset + sizeof(uint2) * 17 * kGroup;
+struct LaneLdsPal {  // 148 032 bytes: one 768-thread workgroup per CU
+    LaneHead head;
+    LanePal pal;
+    uint2 sums[17][kQueueThreads];
+};
+struct LaneLdsPlain {  // 37 440 bytes: four 256-thread workgroups per CU
+    LaneHead head;
+    uint2 sums[17][kThreads];
 };
 static_assert(sizeof(LaneHead) % 16 == 0 && sizeof(LanePal) % 16 == 0, "LDS sections stay 16-byte aligned");
-__device__ __forceinline__ LaneHead &LL() { return *reinterpret_cast<LaneHead *>(qtx_lds_raw); }
-__device__ __forceinline__ LanePal &LP() { return *reinterpret_cast<LanePal *>(qtx_lds_raw + sizeof(LaneHead)); }
+__shared__ LaneLdsPal lane_lds_pal;
+__shared__ LaneLdsPlain lane_lds_plain;
+template <bool PAL> __device__ __forceinline__ LaneHead &LL() {
+    if constexpr (PAL) return lane_lds_pal.head;
+    else return lane_lds_plain.head;
+}
+__device__ __forceinline__ LanePal &LP() { return lane_lds_pal.pal; }
 // this thread's sum of its first k sorted texels
 template <bool PAL> __device__ __forceinline__ uint2 &lane_sum(uint32_t k) {
-    return reinterpret_cast<uint2 *>(qtx_lds_raw + LaneLayout<PAL>::kSumsOffset)[k * LaneLayout<PAL>::kGroup + threadIdx.x];
+    if constexpr (PAL) return lane_lds_pal.sums[k][threadIdx.x];
+    else return lane_lds_plain.sums[k][threadIdx.x];
 }
 
 // Optimisation barrier on the texel registers (emits no instruction): values the
@@ -70,9 +82,9 @@ __device__ __forceinline__ void touch(uint32_t (&px)[16]) {
 }
 
 template <int INTERP, bool PAL> __device__ __forceinline__ void load_tables(const DeviceTables *T) {
-    for (int i = (int)threadIdx.x; i < 512; i += (int)blockDim.x) LL().match[i >> 8][i & 255] = T->match[INTERP][i >> 8][i & 255];
-    if (threadIdx.x < 32) LL().mid5[threadIdx.x] = T->mid5[threadIdx.x];
-    if (threadIdx.x < 64) LL().mid6[threadIdx.x] = T->mid6[threadIdx.x];
+    for (int i = (int)threadIdx.x; i < 512; i += (int)blockDim.x) LL<PAL>().match[i >> 8][i & 255] = T->match[INTERP][i >> 8][i & 255];
+    if (threadIdx.x < 32) LL<PAL>().mid5[threadIdx.x] = T->mid5[threadIdx.x];
+    if (threadIdx.x < 64) LL<PAL>().mid6[threadIdx.x] = T->mid6[threadIdx.x];
     lane_sum<PAL>(0) = make_uint2(0, 0);
     if (PAL) {
         // computed by palette() itself, channel by channel (its channels are
@@ -134,13 +146,14 @@ __device__ __forceinline__ void lane_palette_parts(const uint32_t (&e)[6], bool 
 }
 
 // Color::PreciseRound565 (quicktex/Color.cpp:163-183)
+template <bool PAL>
 __device__ __forceinline__ uint32_t prnd565(float r, float g, float b) {
     int ir = clampi(cvt_trunc(r * 31.0f), 0, 31);
     int ig = clampi(cvt_trunc(g * 63.0f), 0, 63);
     int ib = clampi(cvt_trunc(b * 31.0f), 0, 31);
-    ir += r > LL().mid5[ir];
-    ig += g > LL().mid6[ig];
-    ib += b > LL().mid5[ib];
+    ir += r > LL<PAL>().mid5[ir];
+    ig += g > LL<PAL>().mid6[ig];
+    ib += b > LL<PAL>().mid5[ib];
     return ep_make((uint32_t)ir, (uint32_t)ig, (uint32_t)ib);
 }
 
@@ -151,12 +164,13 @@ __device__ __forceinline__ uint32_t prnd565(float r, float g, float b) {
 // <= 3 x 16 x 255, so |component| < 2^17.  The truncating conversion then needs
 // no out-of-range case, and the six rounding-point reads are issued together.
 // e gets the 5:6:5 components, low r g b then high r g b.
+template <bool PAL>
 __device__ __forceinline__ void prnd565_ls(float lr, float lg, float lb, float hr, float hg, float hb, uint32_t (&e)[6]) {
     const int ilr = clampi((int)(lr * 31.0f), 0, 31), ilg = clampi((int)(lg * 63.0f), 0, 63);
     const int ilb = clampi((int)(lb * 31.0f), 0, 31), ihr = clampi((int)(hr * 31.0f), 0, 31);
     const int ihg = clampi((int)(hg * 63.0f), 0, 63), ihb = clampi((int)(hb * 31.0f), 0, 31);
-    const float mlr = LL().mid5[ilr], mlg = LL().mid6[ilg], mlb = LL().mid5[ilb];
-    const float mhr = LL().mid5[ihr], mhg = LL().mid6[ihg], mhb = LL().mid5[ihb];
+    const float mlr = LL<PAL>().mid5[ilr], mlg = LL<PAL>().mid6[ilg], mlb = LL<PAL>().mid5[ilb];
+    const float mhr = LL<PAL>().mid5[ihr], mhg = LL<PAL>().mid6[ihg], mhb = LL<PAL>().mid5[ihb];
     e[0] = (uint32_t)(ilr + (lr > mlr));
     e[1] = (uint32_t)(ilg + (lg > mlg));
     e[2] = (uint32_t)(ilb + (lb > mlb));
@@ -180,23 +194,25 @@ __device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)
 // FindEndpointsSingleColor(result, color, is_3color) (BC1Encoder.cpp:443-456): the pair
 // from the match tables (the 3-colour tables equal the 4-colour ones); err gets the
 // table error.
+template <bool PAL>
 __device__ __forceinline__ uint32_t match_pair(uint32_t color, uint32_t &err) {
-    const uint32_t mr = LL().match[0][chn(color, 0)];
-    const uint32_t mg = LL().match[1][chn(color, 1)];
-    const uint32_t mb = LL().match[0][chn(color, 2)];
+    const uint32_t mr = LL<PAL>().match[0][chn(color, 0)];
+    const uint32_t mg = LL<PAL>().match[1][chn(color, 1)];
+    const uint32_t mb = LL<PAL>().match[0][chn(color, 2)];
     err = (mr >> 16) + (mg >> 16) + (mb >> 16);
     return ep_make((mr >> 8) & 0xFF, (mg >> 8) & 0xFF, (mb >> 8) & 0xFF) |
            (ep_make(mr & 0xFF, mg & 0xFF, mb & 0xFF) << 16);
 }
 
 // WriteBlockSolid (BC1Encoder.cpp:345-393)
+template <bool PAL>
 __device__ __forceinline__ uint2 write_solid(uint32_t color) {
     uint32_t mask = 0xAA, min16, max16;
     if ((chn(color, 0) | chn(color, 1) | chn(color, 2)) == 0) {
         min16 = 0; max16 = 1; mask = 0x55;
     } else {
         uint32_t e;
-        const uint32_t ep = match_pair(color, e);
+        const uint32_t ep = match_pair<PAL>(color, e);
         // the 3-colour attempt uses identical tables, so it never wins (:359-364)
         min16 = ep & 0xFFFFu;
         max16 = ep >> 16;
@@ -244,6 +260,7 @@ __device__ __forceinline__ Metrics metrics(const uint32_t (&px)[16], bool ignore
 }
 
 // FindEndpoints (BC1Encoder.cpp:473-677).  Returns low | high << 16.
+template <bool PAL>
 __device__ __forceinline__ uint32_t find_endpoints(uint32_t (&px)[16], const Metrics &m, int endpoint_mode,
                                                    bool ignore_black, uint32_t power_iterations) {
     const uint32_t mn0 = chn(m.mn, 0), mn1 = chn(m.mn, 1), mn2 = chn(m.mn, 2);
@@ -305,7 +322,7 @@ __device__ __forceinline__ uint32_t find_endpoints(uint32_t (&px)[16], const Met
         const float hr = chan0 == 0 ? xh : (chan1 == 0 ? y1h : y2h);
         const float hg = chan0 == 1 ? xh : (chan1 == 1 ? y1h : y2h);
         const float hb = chan0 == 2 ? xh : (chan1 == 2 ? y1h : y2h);
-        return prnd565(lr, lg, lb) | (prnd565(hr, hg, hb) << 16);
+        return prnd565<PAL>(lr, lg, lb) | (prnd565<PAL>(hr, hg, hb) << 16);
     }
     if (endpoint_mode == QTX_EP_BOUNDING_BOX || endpoint_mode == QTX_EP_BOUNDING_BOX_INT) {
         touch(px);
@@ -329,7 +346,7 @@ __device__ __forceinline__ uint32_t find_endpoints(uint32_t (&px)[16], const Met
             l2 += i2; h2 -= i2;
             if (icov_xz < 0) { const float t = l0; l0 = h0; h0 = t; }
             if (icov_yz < 0) { const float t = l1; l1 = h1; h1 = t; }
-            return prnd565(l0, l1, l2) | (prnd565(h0, h1, h2) << 16);
+            return prnd565<PAL>(l0, l1, l2) | (prnd565<PAL>(h0, h1, h2) << 16);
         }
         // :581-605 (arithmetic shift of a negative inset, clamp255)
         const int in0 = ((int)(mx0 - mn0) - 8) >> 4, in1 = ((int)(mx1 - mn1) - 8) >> 4, in2 = ((int)(mx2 - mn2) - 8) >> 4;
@@ -566,7 +583,7 @@ __device__ __forceinline__ Res single_color_px(const uint32_t (&px)[16], const B
     for (int i = 0; i < 16; i++) pq += dot4(px[i], qc);
     uint32_t e;
     Res r;
-    r.ep = match_pair(color, e);
+    r.ep = match_pair<PAL>(color, e);
     r.err = bk.pp - 2u * pq + 16u * dot4(qc, qc) + bk.aa;
     r.sel = 0x55555555u;
     r.mode = three ? QTX_COLOR_THREE : QTX_COLOR_FOUR;
@@ -575,6 +592,7 @@ __device__ __forceinline__ Res single_color_px(const uint32_t (&px)[16], const B
 
 // RefineEndpointsLS(result, pixels, metrics) with 4-colour weights (BC1Encoder.cpp:781-825;
 // RefineBlockLS always instantiates it <FourColor>, :860).  0xFFFFFFFF when singular.
+template <bool PAL>
 __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uint32_t sel, const Metrics &m) {
     uint32_t qa = 0, qb = 0;
 #pragma unroll
@@ -597,7 +615,7 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uin
     const float q0 = (float)(qa & 0xFFFFu), q1 = (float)(qa >> 16), q2 = (float)qb;
     const float r0 = (float)(m.s0 * 3) - q0, r1 = (float)(m.s1 * 3) - q1, r2 = (float)(m.s2 * 3) - q2;
     uint32_t e[6];
-    prnd565_ls((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2), (a2 * q0) + (a3 * r0),
+    prnd565_ls<PAL>((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2), (a2 * q0) + (a3 * r0),
                (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2), e);
     return ep_pair(e);
 }
@@ -609,7 +627,7 @@ __device__ __forceinline__ Res refine_block_ls(uint32_t (&px)[16], const Blk &bk
     for (int pass = 0; pass < passes; pass++) {
         Res t;
         touch(px);
-        const uint32_t ep = ls_from_pixels(px, r.sel, m);
+        const uint32_t ep = ls_from_pixels<PAL>(px, r.sel, m);
         touch(px);
         if (ep == 0xFFFFFFFFu) {
             t = single_color_px<INTERP, PAL>(px, bk, r.ep, m.avg, M != QTX_COLOR_FOUR);
@@ -744,7 +762,7 @@ __device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, co
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
             uint32_t e[6], c[4];
-            prnd565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
+            prnd565_ls<PAL>((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
                        (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
                        (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2), e);
             lane_palette_parts<INTERP, PAL>(e, !four, c);
@@ -935,7 +953,7 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
     bool solid = true;
 #pragma unroll
     for (int i = 1; i < 16; i++) solid &= rgba[i] == rgba[0];  // alpha included (ColorBlock.h:80-86)
-    if (solid) return write_solid(rgba[0]);
+    if (solid) return write_solid<PAL>(rgba[0]);
 
     uint32_t px[16];
     Blk bk{0u, 0u};
@@ -953,7 +971,7 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
     Res orig{0u, 0u, 0xFFFFFFFFu, 0}, result{0u, 0u, 0xFFFFFFFFu, 0};
     for (int round = 0; round < s.ep_passes; round++) {
         const int ep_mode = round == 1 ? QTX_EP_BOUNDING_BOX : s.endpoint_mode;
-        const uint32_t ep = find_endpoints(px, m, ep_mode, false, s.power_iterations);
+        const uint32_t ep = find_endpoints<PAL>(px, m, ep_mode, false, s.power_iterations);
         touch(px);
         Res t{ep, 0u, 0u, QTX_COLOR_FOUR};
         t.err = find_selectors<INTERP, PAL>(px, bk, QTX_COLOR_FOUR, error_mode, ep, t.sel);
@@ -984,7 +1002,7 @@ __device__ __forceinline__ uint2 encode_bc1(const uint32_t (&rgba)[16], const BC
         !((chn(m.mx, 0) | chn(m.mx, 1) | chn(m.mx, 2)) < 4u)) {
         const Metrics mnb = metrics(px, true);
         touch(px);
-        const uint32_t ep = find_endpoints(px, mnb, QTX_EP_PCA, true, s.power_iterations);
+        const uint32_t ep = find_endpoints<PAL>(px, mnb, QTX_EP_PCA, true, s.power_iterations);
         touch(px);
         Res t{ep, 0u, 0u, QTX_COLOR_THREE_BLACK};
         t.err = find_selectors<INTERP, PAL>(px, bk, QTX_COLOR_THREE_BLACK, QTX_ERROR_FULL, ep, t.sel);

@@ -242,7 +242,7 @@ __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const
     lane::load_tables<INTERP, true>(T);
     // the wave's current range and the number of ranges it found used up live in
     // LDS between refills, so that they hold no registers through the searches
-    uint32_t *const ws = lane::LL().wave_state[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+    uint32_t *const ws = lane::LL<true>().wave_state[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
     if ((threadIdx.x & 63) == 0) {
         ws[0] = blockIdx.x % kLaneQueues;
         ws[1] = 0;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const
                         bool solid = true;
 #pragma unroll
                         for (int i = 1; i < 16; i++) solid &= px[i] == px[0];
-                        if (solid) store_lane_block(out, idx, px, lane::write_solid(px[0]), BC3);
+                        if (solid) store_lane_block(out, idx, px, lane::write_solid<true>(px[0]), BC3);
                         else gid = idx;
                     }
                 }
@@ -713,10 +713,8 @@ __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict_
 namespace {
 constexpr int kThreads = 256;
 constexpr size_t kBc1Lds = sizeof(dev::Lds);
-constexpr size_t kLaneLds = dev::lane::LaneLayout<false>::kBytes;
-constexpr size_t kLaneQueueLds = dev::lane::LaneLayout<true>::kBytes;
-// the drain needs the palette tables and the zero row of the prefix sums only
-constexpr size_t kDrainLds = dev::lane::LaneLayout<true>::kSumsOffset + sizeof(uint2) * dev::lane::kQueueThreads;
+// the lane kernels' LDS is static (dev::lane::LaneLdsPal / LaneLdsPlain): no dynamic bytes
+constexpr size_t kLaneLds = 0, kLaneQueueLds = 0, kDrainLds = 0;
 // persistent workgroups for the BC1/BC3 encoder: 256 CUs x 4 resident workgroups
 constexpr int kBc1MaxGrid = 256 * 4;
 inline unsigned grid_for(long long items, int per_block) { return (unsigned)((items + per_block - 1) / per_block); }
