@@ -37,8 +37,8 @@ double bc1_block_ops(const BC1Settings &s);
 // encoder needs enough blocks to fill the GPU, the 16-lane one spreads a block
 // over 16 lanes and wins on few blocks of costly settings.
 constexpr uint64_t kLaneFullBlocks = 262144;  // 256 CUs x 16 waves x 64 lanes
-constexpr uint64_t kLaneMinBlocks = 32768;
-constexpr double kLaneMaxOps = 120000.0;
+constexpr uint64_t kLaneMinBlocks = 32768;    // static lane kernel (settings below kLaneQueueMinOps)
+constexpr double kLaneQueueBaseBlocks = 65536.0;  // queued kernel: from this + ops/2 blocks
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path);
 
 // Work-queue words of the queued lane encoder: kLaneQueues block counters, each

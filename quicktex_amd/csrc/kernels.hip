@@ -733,15 +733,20 @@ double bc1_block_ops(const BC1Settings &s) {
     return ops;
 }
 
-// Crossover measured on MI355X (ThreeColorBlack, block mixture): the lane encoder
-// wins from 65k blocks at levels <= 15 and from 262k blocks at level 18.
+// Crossovers measured on MI355X (ThreeColorBlack, block mixture, Ideal): the static
+// lane kernel (levels 0-9) wins from 32k blocks; the queued kernel, whose waves run
+// one whole block per lane (a fixed cost of one block's latency), from about
+// 65k + ops/2 blocks: 81k at level 10, 93k at level 13, 125k at level 16, 150k at
+// level 18.
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
     const int p = path != QTX_BC1_AUTO ? path : bc1_path_env();
     if (p == QTX_BC1_GROUP) return false;
     if (p == QTX_BC1_LANE) return dev::lane::lane_settings(s.exhaustive);
     if (!dev::lane::lane_settings(s.exhaustive)) return false;
     if (blocks >= kLaneFullBlocks) return true;
-    return blocks >= kLaneMinBlocks && bc1_block_ops(s) <= kLaneMaxOps;
+    const double ops = bc1_block_ops(s);
+    if (ops < kLaneQueueMinOps) return blocks >= kLaneMinBlocks;
+    return (double)blocks >= kLaneQueueBaseBlocks + 0.5 * ops;
 }
 
 bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
