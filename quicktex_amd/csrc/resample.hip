@@ -254,6 +254,13 @@ __global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__r
             const int G = p.gx[m], gshift = __builtin_ctz((unsigned)G);
             const int32_t *kt = wsi + p.cx[m];
             const int32_t *win = wsi + p.wx[m];
+            // A shrink by r = 2^a (a <= 10): an interior window holds 2r taps whose
+            // weights are (t + 1/2) / r^2 rising and mirrored falling -- Pillow's
+            // (2t + 1) << (21 - 2a) in 22-bit fixed point, exactly -- so those
+            // windows take their weights from t instead of the table.
+            const int r = p.w / lw;
+            const bool closed = ((p.exact_x >> m) & 1u) && r <= 1024;
+            const int sh = closed ? 21 - 2 * __builtin_ctz((unsigned)r) : 0;
             uint32_t *dst = reinterpret_cast<uint32_t *>(p.lh[m] != p.h ? ws + p.tmp[m] : p.dst[m]) + (size_t)y * lw;
             const int lanes = lw * G;
             for (int b = 0; b < lanes; b += 256) {  // uniform across the workgroup
@@ -263,9 +270,15 @@ __global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__r
                 int32_t ss[4] = {0, 0, 0, 0};
                 if (act) {
                     const int lo = win[2 * o], n = win[2 * o + 1];
-                    // unrolled so that several coefficient loads are in flight
+                    if (closed && n == 2 * r) {
 #pragma unroll 8
-                    for (int t = l; t < n; t += G) accumulate(row[lo + t], kt[(size_t)t * lw + o], ss);
+                        for (int t = l; t < n; t += G)
+                            accumulate(row[lo + t], (t < r ? 2 * t + 1 : 4 * r - 2 * t - 1) << sh, ss);
+                    } else {
+                        // unrolled so that several coefficient loads are in flight
+#pragma unroll 8
+                        for (int t = l; t < n; t += G) accumulate(row[lo + t], kt[(size_t)t * lw + o], ss);
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < 4; c++) ss[c] = group_sum(ss[c], G);  // every lane takes part
