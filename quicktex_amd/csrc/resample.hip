@@ -232,10 +232,10 @@ __global__ __launch_bounds__(256) void k_mip_pass(MipPlan p, const uint8_t *__re
 
 // The horizontal pass of every level for whole base rows: a workgroup copies a
 // row into LDS once and computes that row of every level from it.  Each output
-// of level m is summed by a group of G = gx[m] lanes (1, or 16 for windows of 32
-// taps and more), lane l of the group taking taps l, l + G, ...; the group adds
-// its partial sums with DPP lane moves (integer sums: any order gives the same
-// total).
+// of level m is summed by a group of G = gx[m] lanes (1; 16 for windows of 32
+// taps and more; 64 for windows of 512 and more), lane l of the group taking taps
+// l, l + G, ...; the group adds its partial sums with DPP lane moves (integer
+// sums: any order gives the same total).
 // Rows of at most kRowMaxWidth texels.
 extern __shared__ __align__(16) uint32_t mip_lds[];
 
@@ -382,8 +382,11 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
         if (power_of_two_scale(w, p.lw[m])) p.exact_x |= 1u << m;
         // one lane per output for short windows (consecutive outputs read LDS words
         // `scale` apart: at most 8-way bank conflicts), a 16-lane group for long
-        // ones (a group reads 16 consecutive words)
-        p.gx[m] = (double)w / (double)p.lw[m] >= 16.0 ? 16 : 1;
+        // ones (a group reads 16 consecutive words), a whole wave for windows of 512
+        // taps and more (the few outputs of the last levels, whose taps would
+        // otherwise queue up on a handful of lanes)
+        const double scale_x = (double)w / (double)p.lw[m];
+        p.gx[m] = scale_x >= 256.0 ? 64 : scale_x >= 16.0 ? 16 : 1;
         if (power_of_two_scale(h, p.lh[m])) p.exact_y |= 1u << m;
     }
     uint32_t coef = 0, hu = 0, vu = 0;
