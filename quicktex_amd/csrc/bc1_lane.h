@@ -191,6 +191,15 @@ __device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
 }
 __device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
 
+// ErrorMode::Check2's scale of the projection on the axis (BC1Encoder.cpp:721-749):
+// 4 / (|axis|^2 + 1.25e-6).  It is below 4 unless the axis is zero, and then the
+// projection (p - c0) . axis is zero too; projections are at most 2 x 3 x 255^2 in
+// magnitude, so projection x f + 0.5 always converts to int in range, and the
+// truncating conversion needs no out-of-range case.
+__device__ __forceinline__ float check2_f(int a0, int a1, int a2) {
+    return 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+}
+
 // FindEndpointsSingleColor(result, color, is_3color) (BC1Encoder.cpp:443-456): the pair
 // from the match tables (the 3-colour tables equal the 4-colour ones); err gets the
 // table error.
@@ -457,15 +466,18 @@ __device__ __forceinline__ uint32_t find_selectors_c(const uint32_t (&px)[16], c
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+        const float f = check2_f(a0, a1, a2);
         const int c0a = (int)dot4(c[0], c[3]) - cc0;
+        const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
+        const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t p = px[i];
             const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
             const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-            const int k = clampi(cvt_trunc((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);
-            const int u0 = 2 * d0 - cc0, u1 = 2 * d1 - cc1, u2 = 2 * d2 - cc2, u3 = 2 * d3 - cc3;
+            const int k = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+            const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
+            const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
             const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);  // candidate k-1
             const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);  // candidate k
             uint32_t s = (uint32_t)k;
@@ -534,15 +546,18 @@ __device__ __forceinline__ uint32_t trial_error_c(const uint32_t (&px)[16], cons
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+        const float f = check2_f(a0, a1, a2);
         const int c0a = (int)dot4(c[0], c[3]) - cc0;
+        const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
+        const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t p = px[i];
             const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
             const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-            const int k = clampi(cvt_trunc((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);
-            const int u0 = 2 * d0 - cc0, u1 = 2 * d1 - cc1, u2 = 2 * d2 - cc2, u3 = 2 * d3 - cc3;
+            const int k = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+            const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
+            const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
             const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);
             const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);
             tot += max(ua, ub);
