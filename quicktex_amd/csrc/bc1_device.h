@@ -276,6 +276,24 @@ __device__ __forceinline__ uint32_t precise_round565(float r, float g, float b) 
 // trial (they all require a strictly lower error), so the full error decides the
 // same way.
 
+// -v, opaque to the optimiser: 2 p.c - c.c is then written (p.c << 1) + (-c.c), one
+// v_lshl_add_u32, where the folded form 2 p.c - c.c costs a shift and a subtract.
+__device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
+    uint32_t n = 0u - v;
+    asm("" : "+v"(n));
+    return n;
+}
+__device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
+
+// ErrorMode::Check2's scale of the projection on the axis (BC1Encoder.cpp:721-749):
+// 4 / (|axis|^2 + 1.25e-6).  It is below 4 unless the axis is zero, and then the
+// projection (p - c0) . axis is zero too; projections are at most 2 x 3 x 255^2 in
+// magnitude, so projection x f + 0.5 always converts to int in range, and the
+// truncating conversion needs no out-of-range case.
+__device__ __forceinline__ float check2_f(int a0, int a1, int a2) {
+    return 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+}
+
 // Texel-parallel, with selectors: returns (error, selectors).
 template <int INTERP>
 __device__ __forceinline__ uint2 eval_px(const Ctx &cx, int M, int error_mode, uint32_t ep) {
@@ -333,6 +351,8 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
     palette_pair<INTERP>(ep, (M & 0xF) == 3, c);
     const int cc0 = (int)dot4(c[0], c[0]), cc1 = (int)dot4(c[1], c[1]);
     const int cc2 = (int)dot4(c[2], c[2]), cc3 = (int)dot4(c[3], c[3]);
+    const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
+    const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
     int tot = 0;
     if (error_mode == QTX_ERROR_FULL) {
         if (M == QTX_COLOR_THREE) {
@@ -341,9 +361,8 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t p = pick(t, k);
-                    const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
-                    const int u2 = 2 * (int)dot4(p, c[2]) - cc2;
-                    tot += max(max(u0, u1), u2);
+                    tot += max(max(twice_plus(dot4(p, c[0]), n0), twice_plus(dot4(p, c[1]), n1)),
+                               twice_plus(dot4(p, c[2]), n2));
                 }
             }
         } else {
@@ -352,8 +371,8 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t p = pick(t, k);
-                    const int u0 = 2 * (int)dot4(p, c[0]) - cc0, u1 = 2 * (int)dot4(p, c[1]) - cc1;
-                    const int u2 = 2 * (int)dot4(p, c[2]) - cc2, u3 = 2 * (int)dot4(p, c[3]) - cc3;
+                    const int u0 = twice_plus(dot4(p, c[0]), n0), u1 = twice_plus(dot4(p, c[1]), n1);
+                    const int u2 = twice_plus(dot4(p, c[2]), n2), u3 = twice_plus(dot4(p, c[3]), n3);
                     tot += max(max(u0, u1), max(u2, u3));
                 }
             }
@@ -373,8 +392,9 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
                 const uint32_t p = pick(t, k);
                 const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
                 const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-                const int s = clampi(cvt_trunc((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);
-                const int u0 = 2 * d0 - cc0, u1 = 2 * d1 - cc1, u2 = 2 * d2 - cc2, u3 = 2 * d3 - cc3;
+                const int s = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+                const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
+                const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
                 const int ua = s == 1 ? u0 : (s == 2 ? u1 : u2);
                 const int ub = s == 1 ? u1 : (s == 2 ? u2 : u3);
                 tot += max(ua, ub);

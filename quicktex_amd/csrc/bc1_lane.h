@@ -182,24 +182,6 @@ __device__ __forceinline__ uint32_t ep_pair(const uint32_t (&e)[6]) {
     return ep_make(e[0], e[1], e[2]) | (ep_make(e[3], e[4], e[5]) << 16);
 }
 
-// -v, opaque to the optimiser: 2 p.c - c.c is then written (p.c << 1) + (-c.c), one
-// v_lshl_add_u32, where the folded form 2 p.c - c.c costs a shift and a subtract.
-__device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
-    uint32_t n = 0u - v;
-    asm("" : "+v"(n));
-    return n;
-}
-__device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
-
-// ErrorMode::Check2's scale of the projection on the axis (BC1Encoder.cpp:721-749):
-// 4 / (|axis|^2 + 1.25e-6).  It is below 4 unless the axis is zero, and then the
-// projection (p - c0) . axis is zero too; projections are at most 2 x 3 x 255^2 in
-// magnitude, so projection x f + 0.5 always converts to int in range, and the
-// truncating conversion needs no out-of-range case.
-__device__ __forceinline__ float check2_f(int a0, int a1, int a2) {
-    return 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
-}
-
 // FindEndpointsSingleColor(result, color, is_3color) (BC1Encoder.cpp:443-456): the pair
 // from the match tables (the 3-colour tables equal the 4-colour ones); err gets the
 // table error.
