@@ -270,6 +270,23 @@ __device__ __forceinline__ uint32_t precise_round565(float r, float g, float b) 
     return ep_make((uint32_t)ir, (uint32_t)ig, (uint32_t)ib);
 }
 
+// PreciseRound565 of both endpoints of a least-squares solve (low | high << 16).
+// Every component there is bounded far inside the int range (coefficients: an
+// adjugate of integer weight sums over a non-zero integer determinant, times 3/255
+// or 2/255, so below 2 in magnitude; texel sums at most 3 x 16 x 255), so the
+// truncating conversion needs no out-of-range case, and the six rounding-point
+// reads go out together.
+__device__ __forceinline__ uint32_t precise_round565_ls(float lr, float lg, float lb, float hr, float hg, float hb) {
+    const int ilr = clampi((int)(lr * 31.0f), 0, 31), ilg = clampi((int)(lg * 63.0f), 0, 63);
+    const int ilb = clampi((int)(lb * 31.0f), 0, 31), ihr = clampi((int)(hr * 31.0f), 0, 31);
+    const int ihg = clampi((int)(hg * 63.0f), 0, 63), ihb = clampi((int)(hb * 31.0f), 0, 31);
+    const float mlr = L().mid5[ilr], mlg = L().mid6[ilg], mlb = L().mid5[ilb];
+    const float mhr = L().mid5[ihr], mhg = L().mid6[ihg], mhb = L().mid5[ihb];
+    const uint32_t lo = ep_make((uint32_t)(ilr + (lr > mlr)), (uint32_t)(ilg + (lg > mlg)), (uint32_t)(ilb + (lb > mlb)));
+    const uint32_t hi = ep_make((uint32_t)(ihr + (hr > mhr)), (uint32_t)(ihg + (hg > mhg)), (uint32_t)(ihb + (hb > mhb)));
+    return lo | (hi << 16);
+}
+
 // ---------------------------------------------------------------------------
 // Selector search, FindSelectors<M> (BC1Encoder.cpp:679-779).  The reference stops
 // early once the running error reaches the bound; no caller ever accepts such a
@@ -480,9 +497,8 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const Ctx &cx, uint32_t sel, 
     const float a0 = (m3 * 1.0f) * scale, a1 = (m1 * -1.0f) * scale, a2 = (m1 * -1.0f) * scale, a3 = (m0 * 1.0f) * scale;
     const float q0 = (float)(qa & 0xFFFFu), q1 = (float)(qa >> 16), q2 = (float)qb;
     const float r0 = (float)(m.s0 * 3) - q0, r1 = (float)(m.s1 * 3) - q1, r2 = (float)(m.s2 * 3) - q2;
-    const uint32_t lo = precise_round565((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2));
-    const uint32_t hi = precise_round565((a2 * q0) + (a3 * r0), (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2));
-    return lo | (hi << 16);
+    return precise_round565_ls((a0 * q0) + (a1 * r0), (a0 * q1) + (a1 * r1), (a0 * q2) + (a1 * r2),
+                               (a2 * q0) + (a3 * r0), (a2 * q1) + (a3 * r1), (a2 * q2) + (a3 * r2));
 }
 
 // RefineBlockLS<M> (BC1Encoder.cpp:852-875)
@@ -712,11 +728,9 @@ __device__ __forceinline__ Res cluster_fit(const Ctx &cx, const Metrics &m, int 
             // q10 = sums[l1] + sums[l2] (+ sums[l3]), q00 = d * sums[16] - q10: exact integers
             const float q10_0 = (float)(qa & 0xFFFFu), q10_1 = (float)(qa >> 16), q10_2 = (float)qb;
             const float q00_0 = (s16_0 * den) - q10_0, q00_1 = (s16_1 * den) - q10_1, q00_2 = (s16_2 * den) - q10_2;
-            const uint32_t lo = precise_round565((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
-                                                 (fm.x * q00_2) + (fm.y * q10_2));
-            const uint32_t hi = precise_round565((fm.z * q00_0) + (fm.w * q10_0), (fm.z * q00_1) + (fm.w * q10_1),
-                                                 (fm.z * q00_2) + (fm.w * q10_2));
-            ep = lo | (hi << 16);
+            ep = precise_round565_ls((fm.x * q00_0) + (fm.y * q10_0), (fm.x * q00_1) + (fm.y * q10_1),
+                                     (fm.x * q00_2) + (fm.y * q10_2), (fm.z * q00_0) + (fm.w * q10_0),
+                                     (fm.z * q00_1) + (fm.w * q10_1), (fm.z * q00_2) + (fm.w * q10_2));
             err = trial_err<INTERP>(cx, M, error_mode, ep);
         }
         if (err < b_err) {  // ties keep this lane's earliest ordering
