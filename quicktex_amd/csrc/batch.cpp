@@ -119,7 +119,7 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
     out.host.clear();
     out.groups.clear();
     // BC1/BC3 jobs of an interpolator go to the lane encoder when there is enough
-    // such work in the launch (use_lane_encoder with the group's total blocks)
+    // such work in the launch (use_lane_batch with the group's total blocks)
     uint64_t colour_blocks[4] = {0, 0, 0, 0};
     for (const Prepared &p : jobs)
         if (p.group >= kGroupColour && p.group < kGroupLane) colour_blocks[p.group - kGroupColour] += p.job.nblocks;
@@ -127,7 +127,7 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
     for (size_t i = 0; i < jobs.size(); i++) {
         const int g = jobs[i].group;
         group[i] = g;
-        if (g >= kGroupColour && g < kGroupLane && use_lane_encoder(jobs[i].job.s, colour_blocks[g - kGroupColour], bc1_path))
+        if (g >= kGroupColour && g < kGroupLane && use_lane_batch(jobs[i].job.s, colour_blocks[g - kGroupColour], bc1_path))
             group[i] = kGroupLane + (g - kGroupColour);
     }
     std::vector<int> idx(jobs.size());

@@ -40,6 +40,10 @@ constexpr uint64_t kLaneFullBlocks = 262144;  // 256 CUs x 16 waves x 64 lanes
 constexpr uint64_t kLaneMinBlocks = 32768;    // static lane kernel (settings below kLaneQueueMinOps)
 constexpr double kLaneQueueBaseBlocks = 65536.0;  // queued kernel: from this + ops/2 blocks
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path);
+// The same choice for a batch's BC1/BC3 jobs (blocks: the batch's total of such
+// work), whose lane kernel is the static one.
+constexpr double kLaneBatchMaxOps = 120000.0;
+bool use_lane_batch(const BC1Settings &s, uint64_t blocks, int path);
 
 // Work-queue words of the queued lane encoder: kLaneQueues block counters, each
 // on its own 64-byte line, and a count of finished waves.

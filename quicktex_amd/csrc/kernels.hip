@@ -749,6 +749,17 @@ bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
     return (double)blocks >= kLaneQueueBaseBlocks + 0.5 * ops;
 }
 
+// The batch's lane kernel (k_lane_batch) deals no blocks and parks no searches:
+// it wins from 32k blocks at settings up to 120 k ops/block, from 262k above.
+bool use_lane_batch(const BC1Settings &s, uint64_t blocks, int path) {
+    const int p = path != QTX_BC1_AUTO ? path : bc1_path_env();
+    if (p == QTX_BC1_GROUP) return false;
+    if (p == QTX_BC1_LANE) return dev::lane::lane_settings(s.exhaustive);
+    if (!dev::lane::lane_settings(s.exhaustive)) return false;
+    if (blocks >= kLaneFullBlocks) return true;
+    return blocks >= kLaneMinBlocks && bc1_block_ops(s) <= kLaneBatchMaxOps;
+}
+
 bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
     const uint64_t nblocks = (uint64_t)((w + 3) / 4) * (uint64_t)((h + 3) / 4);
     return use_lane_encoder(s, nblocks, path) && lane_queue_env() && bc1_block_ops(s) >= kLaneQueueMinOps;
