@@ -7,7 +7,10 @@ blocks (quicktex_amd/synth.py) -- in ThreeColorBlack mode, once per
 interpolator (Ideal, IdealRound, Nvidia, AMD), one plain encode per texture
 (the queued lane kernel, then the drain kernel for the endpoint searches it
 parked).  One step = the four encodes, inputs already resident in HBM.  value =
-Mpixel/s over the timed steps.
+Mpixel/s over the timed steps.  sustained: --sustain-steps more steps back to
+back; under continuous load of this VALU-bound kernel the clocks come down after
+a fraction of a second, and the last half of those steps gives the sustained
+rate.
 
 roofline: the encode against the VALU peak (256 CU x 4 SIMD x 16 lanes x
 2.4 GHz = 39.3 T lane-ops/s; the encoder is integer/FP32 ALU work, no HBM or
@@ -59,7 +62,7 @@ INTERPS = ('Ideal', 'IdealRound', 'Nvidia', 'AMD')
 # HBM bytes per BC1 encode on this workload (k_bc1_lane_queue + k_bc1_search_drain),
 # from the committed counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2 +
 # WRITE_SIZE x 1 (copy-kernel calibration), mean over the four interpolators.
-TRAFFIC_BYTES = (433.48e6 + 432.73e6 + 453.84e6 + 440.27e6) / 4
+TRAFFIC_BYTES = (644.49e6 + 642.71e6 + 666.80e6 + 649.27e6) / 4
 
 
 def log(*a):
@@ -116,7 +119,28 @@ def plain_leg(a, q, hip, np):
     value = 4 * W * H / (step_ms / 1000.0) / 1e6
     per_interp = {INTERPS[i]: round(W * H / (dur[:, i].mean() / 1000.0) / 1e6, 1) for i in range(4)}
     log('step %.2f ms, %.1f Mpx/s' % (step_ms, value))
+    log('mean encode per step (ms): ' + ' '.join('%.3f' % m for m in dur.mean(axis=1)))
     gpu_out = d_out.download().reshape(4, nb, 8)
+
+    # sustained load: the same steps back to back for a while longer; under
+    # continuous load the clocks come down after a fraction of a second
+    sustained = None
+    if a.sustain_steps > 0:
+        sev = [hip.Event() for _ in range(a.sustain_steps + 1)]
+        sev[0].record(stream)
+        for s in range(a.sustain_steps):
+            for i in range(4):
+                launch(i)
+            sev[s + 1].record(stream)
+        stream.synchronize()
+        per_step = np.array([sev[s].elapsed_ms(sev[s + 1]) for s in range(a.sustain_steps)])
+        tail = per_step[a.sustain_steps // 2:]
+        sustained = {'steps': a.sustain_steps, 'seconds': round(float(per_step.sum()) / 1000.0, 3),
+                     'first_step_ms': round(float(per_step[0]), 3),
+                     'last_half_ms_per_step': round(float(tail.mean()), 3),
+                     'last_half_mpx_s': round(4 * W * H / (float(tail.mean()) / 1000.0) / 1e6, 1)}
+        log('sustained: %d steps in %.2f s, last half %.3f ms per step (%.1f Mpx/s)' % (
+            a.sustain_steps, sustained['seconds'], sustained['last_half_ms_per_step'], sustained['last_half_mpx_s']))
 
     # algorithmic ops: oracle accounting on a 16-row sample, scaled per launch
     from tests._cpu_libs import op_count
@@ -162,7 +186,7 @@ def plain_leg(a, q, hip, np):
                    rows, H, nbp, secs)}
         log('cpu %s: %.3f Mpx/s, bit-exact on sample: %s' % (kind, cpu_mpx, exact))
     stream.destroy()
-    return value, step_ms, per_interp, roofline, cpu, exact
+    return value, step_ms, per_interp, roofline, cpu, exact, sustained
 
 
 # ---------------------------------------------------------------------------
@@ -636,6 +660,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--sustain-steps', type=int, default=200,
+                    help='plain leg: steps run back to back after the timed ones (0: none)')
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
@@ -652,6 +678,8 @@ def main():
     ap.add_argument('--dds-cpu-size', type=int, default=1024, help='base side of the dds leg\'s CPU sample')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run 3 device-to-device copies of MIB MiB (calibrates the HBM byte counters) and exit')
+    ap.add_argument('--level-probe', type=int, default=-1, metavar='LEVEL',
+                    help='only run 3 BC1 encodes of the 4096x4096 texture at LEVEL (FourColor, Ideal; for counter runs) and exit')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: the codec has no multi-GPU path (textures shard with no '
@@ -675,6 +703,19 @@ def main():
             dst.copy_from(src)
         print(json.dumps({'copy_probe_bytes': n, 'copies': 3}), flush=True)
         return
+    if a.level_probe >= 0:
+        from quicktex_amd import synth
+        from quicktex_amd.capi import COLOR_FOUR
+        img = synth.mixture_rgba(W, H, seed=0x5EED0002, near_black=0.10)
+        d_in = hip.Buffer.from_array(img)
+        d_out = hip.Buffer((W // 4) * (H // 4) * 8)
+        stream = hip.Stream()
+        p = q.params(a.level_probe, COLOR_FOUR, 0)
+        for _ in range(3):
+            q.bc1_encode_device(p, d_in.ptr, W, H, d_out.ptr, stream.handle)
+        stream.synchronize()
+        print(json.dumps({'level_probe': a.level_probe, 'encodes': 3}), flush=True)
+        return
 
     line = {'metric': METRIC, 'value': None, 'unit': 'Mpixel/s', 'n_gpus': 1, 'steps': a.steps,
             'warmup': a.warmup, 'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak',
@@ -683,10 +724,10 @@ def main():
                        'texture': '%dx%d' % (W, H), 'level': 18, 'color_mode': 'ThreeColorBlack',
                        'interpolators': list(INTERPS)}}
     if 'plain' in legs:
-        value, step_ms, per_interp, roofline, cpu, exact = plain_leg(a, q, hip, np)
+        value, step_ms, per_interp, roofline, cpu, exact, sustained = plain_leg(a, q, hip, np)
         line.update({'value': round(value, 1), 'ms_per_step': round(step_ms, 3),
                      'per_interpolator_mpx_s': per_interp, 'bit_exact_vs_cpu_sample': exact,
-                     'roofline': roofline, 'cpu_baseline': cpu})
+                     'roofline': roofline, 'cpu_baseline': cpu, 'sustained': sustained})
         if cpu:
             line['gpu_vs_cpu'] = round(value / cpu['value'], 1)
     if 'batched' in legs:

@@ -65,7 +65,7 @@ bool wants_lane_queue(const BC1Settings &s, int w, int h, int path);
 // long search never holds 63 idle lanes.  Most searches end at round 33 or 34 (no
 // improvement); the park words count the parked searches and the drain's
 // finished waves.
-constexpr uint32_t kParkStop = 40;
+constexpr uint32_t kParkStop = 34;
 constexpr int kParkCountWord = 192;
 constexpr int kParkRetireWord = 193;
 constexpr size_t kParkRecordBytes = 32;
