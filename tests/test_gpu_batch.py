@@ -159,6 +159,39 @@ def test_batch_mip_chain_bc3(qtx, stream, checker):
     b.close()
 
 
+@pytest.mark.parametrize('levels', [(10, 12, 13, 16, 18), (0, 3, 5, 8)])
+def test_lane_batch_forms(qtx, stream, checker, levels):
+    """BC1/BC3 jobs forced onto the batch's lane kernel, at levels with endpoint
+    search and cluster fit (10-18) and without (0-8): ragged sizes, both formats,
+    every interpolator and colour mode, launched twice."""
+    rng = np.random.default_rng(0x1A4E + levels[0])
+    qtx.set_bc1_encoder(1)
+    try:
+        cases = []
+        for k in range(24):
+            w, h = [(1, 1), (37, 53), (130, 18), (64, 64), (256, 96), (5, 300)][k % 6]
+            fmt = FMT_BC3 if k % 3 == 0 else FMT_BC1
+            mode = 4 if fmt == FMT_BC3 else (4, 3, 0x13)[k % 3]
+            p = qtx.params(levels[k % len(levels)], mode, k % 4)
+            img = synth.mixture_rgba(w, h, seed=int(rng.integers(1 << 30)))
+            cases.append((fmt, w, h, p, img))
+        srcs = [hip.Buffer.from_array(img) for (_, _, _, _, img) in cases]
+        outs = _outs_n([nblocks(w, h) * (16 if fmt == FMT_BC3 else 8) for (fmt, w, h, _, _) in cases])
+        jobs = [job(OP_ENCODE, fmt, w, h, s.ptr, o.ptr, params=p) for (fmt, w, h, p, _), s, o in zip(cases, srcs, outs)]
+        b = Batch(qtx, jobs)
+        for rep in range(2):
+            for o in outs:
+                o.zero()
+            b.launch(stream.handle)
+            stream.synchronize()
+            for k, ((fmt, w, h, p, img), o) in enumerate(zip(cases, outs)):
+                want = checker.bc3_encode(img, p) if fmt == FMT_BC3 else checker.bc1_encode(img, p)
+                assert (o.download() == np.asarray(want).reshape(-1)).all(), (rep, k, fmt, w, h)
+        b.close()
+    finally:
+        qtx.set_bc1_encoder(0)
+
+
 def test_batch_errors(qtx, stream):
     d = hip.Buffer(64 * 64 * 4)
     p3 = qtx.params(5, 3, 0)
