@@ -56,18 +56,32 @@ __device__ __forceinline__ uint2 bc4_encode_values(const uint32_t v[16]) {
     return make_uint2(mx | (mn << 8) | ((uint32_t)(bits & 0xFFFFu) << 16), (uint32_t)(bits >> 16));
 }
 
-// BC4Block values (quicktex/s3tc/bc4/BC4Block.cpp:44-64) and selector decode.
-__device__ __forceinline__ uint32_t bc4_value(uint32_t a0, uint32_t a1, uint32_t s) {
-    if (a0 <= a1) {  // 6-value
-        if (s == 0) return a0;
-        if (s == 1) return a1;
-        if (s == 6) return 0;
-        if (s == 7) return 255;
-        return (a0 * (6 - s) + a1 * (s - 1)) / 5;
+// The eight values of a BC4 block (quicktex/s3tc/bc4/BC4Block.cpp:44-64) as
+// bytes, value s in byte s of (lo, hi): both rules computed, one kept, so lanes
+// of either kind do not diverge.
+__device__ __forceinline__ uint2 bc4_palette(uint32_t a0, uint32_t a1) {
+    uint32_t six[4], eight[6];
+#pragma unroll
+    for (uint32_t s = 2; s < 6; s++) six[s - 2] = (a0 * (6 - s) + a1 * (s - 1)) / 5;
+#pragma unroll
+    for (uint32_t s = 2; s < 8; s++) eight[s - 2] = (a0 * (8 - s) + a1 * (s - 1)) / 7;
+    const uint32_t base = a0 | (a1 << 8);
+    return a0 <= a1 ? make_uint2(base | (six[0] << 16) | (six[1] << 24), six[2] | (six[3] << 8) | 0xFF000000u)
+                    : make_uint2(base | (eight[0] << 16) | (eight[1] << 24),
+                                 eight[2] | (eight[3] << 8) | (eight[4] << 16) | (eight[5] << 24));
+}
+
+// The texels' channel ch (byte sh / 8) set from a BC4 block (a0, a1, 48
+// selector bits): each value picked out of the palette bytes by v_perm_b32 and
+// placed at its byte in the same step (selector byte 0x0C gives zero).
+__device__ __forceinline__ void bc4_decode_into(uint32_t px[16], uint32_t a0, uint32_t a1, uint64_t bits, uint32_t sh) {
+    const uint2 pal = bc4_palette(a0, a1);
+    const uint32_t keep = ~(0xFFu << sh), zero = 0x0C0C0C0Cu & keep;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t sel = zero | (((uint32_t)(bits >> (3 * i)) & 7u) << sh);
+        px[i] = (px[i] & keep) | __builtin_amdgcn_perm(pal.y, pal.x, sel);
     }
-    if (s == 0) return a0;
-    if (s == 1) return a1;
-    return (a0 * (8 - s) + a1 * (s - 1)) / 7;
 }
 
 // ---------------------------------------------------------------------------
@@ -515,29 +529,16 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
             const uint32_t c = s == 0 ? bc[0] : (s == 1 ? bc[1] : (s == 2 ? bc[2] : bc[3]));
             px[i] = write_alpha ? c : ((c & 0x00FFFFFFu) | 0xFF000000u);
         }
-        if (KIND == DEC_BC3) {
-            const uint32_t a0 = ab.x & 0xFFu, a1 = (ab.x >> 8) & 0xFFu;
-            const uint64_t bits = (uint64_t)(ab.x >> 16) | ((uint64_t)ab.y << 16);
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t v = bc4_value(a0, a1, (uint32_t)(bits >> (3 * i)) & 7u);
-                px[i] = (px[i] & 0x00FFFFFFu) | (v << 24);
-            }
-        }
+        if (KIND == DEC_BC3)
+            bc4_decode_into(px, ab.x & 0xFFu, (ab.x >> 8) & 0xFFu, (uint64_t)(ab.x >> 16) | ((uint64_t)ab.y << 16), 24u);
     } else {
         const int nch = KIND == DEC_BC5 ? 2 : 1;
 #pragma unroll
         for (int k = 0; k < nch; k++) {
             const uint2 q = reinterpret_cast<const uint2 *>(blocks)[gid * nch + k];
             const int ch = k == 0 ? p0 : p1;
-            const uint32_t a0 = q.x & 0xFFu, a1 = (q.x >> 8) & 0xFFu;
-            const uint64_t bits = (uint64_t)(q.x >> 16) | ((uint64_t)q.y << 16);
-            const uint32_t sh = 8u * (uint32_t)ch;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t v = bc4_value(a0, a1, (uint32_t)(bits >> (3 * i)) & 7u);
-                px[i] = (px[i] & ~(0xFFu << sh)) | (v << sh);
-            }
+            bc4_decode_into(px, q.x & 0xFFu, (q.x >> 8) & 0xFFu, (uint64_t)(q.x >> 16) | ((uint64_t)q.y << 16),
+                            8u * (uint32_t)ch);
         }
     }
     // scatter
