@@ -41,15 +41,20 @@ __device__ __forceinline__ uint2 bc4_encode_values(const uint32_t v[16]) {
         mx = max(mx, v[i]);
     }
     if (mx == mn) return make_uint2(mn | (mn << 8), 0u);
-    const int bias = 4 - (int)mn * 14;
     const int delta = (int)(mx - mn);
+    // The reference's level is the number of c in 0..6 with 14 v >= delta (1 + 2c) -
+    // (4 - 14 mn), that is x >= 2 delta c for x = 14 (v - mn) + 4 - delta: so
+    // floor(x / (2 delta)) + 1, at most 7 (x > -2 delta, so x < 0 gives 0).  The
+    // quotient in float: |x| < 2^12 and r is the correctly rounded 1 / (2 delta);
+    // a quotient that is no integer lies at least 1 / 510 below the next one, so
+    // x r + 2^-11 (one rounding) floors to it exactly -- checked for every delta
+    // and every x on the CPU.
+    const float r = 1.0f / (float)(2 * delta);
     uint64_t bits = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int value = (int)v[i] * 14;
-        int level = 0;
-#pragma unroll
-        for (int c = 0; c < 7; c++) level += value >= delta * (1 + 2 * c) - bias;
+        const int x = 14 * ((int)v[i] - (int)mn) + 4 - delta;
+        const int level = min((int)floorf(fmaf((float)x, r, 0x1p-11f)) + 1, 7);
         const uint64_t s = (0x02345671u >> (4 * level)) & 7u;  // {1,7,6,5,4,3,2,0}
         bits |= s << (3 * i);
     }
