@@ -434,12 +434,10 @@ __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_search_drain(
     }
 }
 
-// BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
-template <bool BC5>
-__device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid, int ch0,
-                                                  int ch1, uint8_t *__restrict__ out) {
+// The 16 texels of block gid (edge blocks wrap like the reference's texel reads).
+__device__ __forceinline__ void load_block_texels(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+                                                  uint32_t px[16]) {
     const int bx = gid % bw, by = gid / bw;
-    uint32_t px[16];
     if ((w & 3) == 0 && bx * 4 + 3 < w && by * 4 + 3 < h) {
 #pragma unroll
         for (int y = 0; y < 4; y++) {
@@ -450,6 +448,14 @@ __device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rg
 #pragma unroll
         for (int t = 0; t < 16; t++) px[t] = load_texel(rgba, w, h, bx, by, t);
     }
+}
+
+// BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
+template <bool BC5>
+__device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid, int ch0,
+                                                  int ch1, uint8_t *__restrict__ out) {
+    uint32_t px[16];
+    load_block_texels(rgba, w, h, bw, gid, px);
     uint32_t v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = chn(px[i], ch0);
@@ -522,17 +528,36 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
         uint32_t pal[4];
         palette<INTERP>(lo, hi, three, pal);
         // back to BC1 selector order {c0, c1, c2, c3}; alpha 255 except 3-colour black
-        uint32_t bc[4];
-        bc[0] = pal[0] | 0xFF000000u;
-        bc[1] = (three ? pal[2] : pal[3]) | 0xFF000000u;
-        bc[2] = pal[1] | 0xFF000000u;
-        bc[3] = three ? 0u : (pal[2] | 0xFF000000u);
+        // (3-colour black keeps alpha 0 only when alpha is written; BC3 takes alpha
+        // from its alpha block below)
         const bool write_alpha = (KIND == DEC_BC1) && p0 != 0;
+        const uint32_t bc0 = pal[0] | 0xFF000000u;
+        const uint32_t bc1 = (three ? pal[2] : pal[3]) | 0xFF000000u;
+        const uint32_t bc2 = pal[1] | 0xFF000000u;
+        const uint32_t bc3 = three ? (write_alpha ? 0u : 0xFF000000u) : (pal[2] | 0xFF000000u);
+        if (KIND == DEC_BC1) {
+            // byte planes: R holds the red of colours 0-3 in bytes 0-3, and so on; a
+            // texel is then two v_perm_b32 (red and green, blue and alpha) of its selector
+            const uint32_t lo01 = __builtin_amdgcn_perm(bc1, bc0, 0x05010400u);
+            const uint32_t hi01 = __builtin_amdgcn_perm(bc1, bc0, 0x07030602u);
+            const uint32_t lo23 = __builtin_amdgcn_perm(bc3, bc2, 0x05010400u);
+            const uint32_t hi23 = __builtin_amdgcn_perm(bc3, bc2, 0x07030602u);
+            const uint32_t R = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+            const uint32_t G = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+            const uint32_t B = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+            const uint32_t A = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t s = (cb.y >> (2 * i)) & 3u;
-            const uint32_t c = s == 0 ? bc[0] : (s == 1 ? bc[1] : (s == 2 ? bc[2] : bc[3]));
-            px[i] = write_alpha ? c : ((c & 0x00FFFFFFu) | 0xFF000000u);
+            for (int i = 0; i < 16; i++) {
+                const uint32_t s = (cb.y >> (2 * i)) & 3u;
+                const uint32_t sel = s * 0x101u + 0x0C0C0400u;  // bytes: s, s + 4, zero, zero
+                px[i] = __builtin_amdgcn_perm(G, R, sel) | __builtin_amdgcn_perm(A, B, (sel << 16) | 0x0C0Cu);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t s = (cb.y >> (2 * i)) & 3u;
+                px[i] = s == 0 ? bc0 : (s == 1 ? bc1 : (s == 2 ? bc2 : bc3));
+            }
         }
         if (KIND == DEC_BC3)
             bc4_decode_into(px, ab.x & 0xFFu, (ab.x >> 8) & 0xFFu, (uint64_t)(ab.x >> 16) | ((uint64_t)ab.y << 16), 24u);
