@@ -935,9 +935,11 @@ template <bool HQ> __device__ __forceinline__ BC1Settings fixed_settings(const B
     return t;
 }
 
-// True when the lane encoder is the faster of the two for these settings: all
-// but the exhaustive cluster fit (level 19), whose 969 + 153 orderings per pass
-// the 16-lane encoder deals to 16 lanes.
+// True when a batch's lane kernel may take these settings: all but the exhaustive
+// cluster fit (level 19), whose 969 + 153 orderings per pass hold a lane for about
+// 4 ms a block -- a small level-19 job in a large batch would set the batch's
+// time -- while the 16-lane encoder deals them to 16 lanes.  (Plain level-19
+// launches of 262k blocks and up take the queued lane kernel.)
 __host__ __device__ constexpr bool lane_settings(int exhaustive) { return exhaustive == 0; }
 
 // BC1Encoder::EncodeBlock (BC1Encoder.cpp:261-342) for one block (texels RGBA8,

@@ -768,12 +768,14 @@ double bc1_block_ops(const BC1Settings &s) {
 // lane kernel (levels 0-9) wins from 32k blocks; the queued kernel, whose waves run
 // one whole block per lane (a fixed cost of one block's latency), from about
 // 65k + ops/2 blocks: 81k at level 10, 93k at level 13, 125k at level 16, 150k at
-// level 18.
+// level 18.  Level 19 (exhaustive cluster fit, about 1 M ops per block): the queued
+// kernel takes 3.7 ms at any size up to 131k blocks (one block per lane) against
+// 0.23-3.0 ms for the 16-lane encoder, 4.8 against 5.9 ms at 262k blocks and 12.0
+// against 20.5 ms at 1 M: from kLaneFullBlocks (the formula gives 565k).
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
     const int p = path != QTX_BC1_AUTO ? path : bc1_path_env();
     if (p == QTX_BC1_GROUP) return false;
-    if (p == QTX_BC1_LANE) return dev::lane::lane_settings(s.exhaustive);
-    if (!dev::lane::lane_settings(s.exhaustive)) return false;
+    if (p == QTX_BC1_LANE) return true;
     if (blocks >= kLaneFullBlocks) return true;
     const double ops = bc1_block_ops(s);
     if (ops < kLaneQueueMinOps) return blocks >= kLaneMinBlocks;
