@@ -9,7 +9,9 @@ kernel takes over), and the dealing's edge cases: all blocks solid, none solid,
 one block, ragged sizes.  Endpoint searches that run past the park stop are
 parked by the queued kernel and finished by the drain kernel: checked at level
 18 for every interpolator, BC1 and BC3, relaunched on one stream (the park words
-return to zero) and in flight on several streams (one park buffer per set).
+return to zero) and in flight on several streams (one park buffer per set), and
+at level 19 on a texture large enough for the automatic choice to take the queued
+kernel (against the 16-lane encoder's bytes).
 """
 import numpy as np
 import pytest
@@ -169,3 +171,24 @@ def test_parked_relaunch_and_streams(lane, checker):
         assert (d_out.download().reshape(-1) == want).all(), 'stream %d' % i
     for st in streams:
         st.destroy()
+
+
+@pytest.mark.parametrize('interp', [0, 2])
+def test_level19_large_texture_both_encoders(qtx, interp):
+    """Level 19 (exhaustive cluster fit) on a 2048x2048 texture (262144 blocks):
+    routed automatically to the queued lane kernel with parking and the drain;
+    its bytes must equal the 16-lane encoder's (which the parity tests check
+    against the CPU at every level on smaller textures)."""
+    from quicktex_amd import hip
+    img = synth.mixture_rgba(2048, 2048, seed=70 + interp, near_black=0.1)
+    p = qtx.params(19, 0x13, interp)
+    s = hip.Stream()
+    outs = {}
+    for enc in (0, 2):  # automatic (lane at this size), 16-lane
+        qtx.set_bc1_encoder(enc)
+        d_in, d_out = _encode_device(qtx, hip, img, p, s)
+        s.synchronize()
+        outs[enc] = d_out.download()
+    qtx.set_bc1_encoder(0)
+    s.destroy()
+    assert (outs[0] == outs[2]).all()
