@@ -92,7 +92,8 @@ qtx_status qtx_ctx_synchronize(qtx_ctx *ctx);
 
 /* Which of the two BC1/BC3 encoders serves this context's encodes: QTX_BC1_AUTO
  * (default) picks by setting and amount of work, QTX_BC1_LANE forces the
- * one-block-per-lane encoder (any non-exhaustive setting), QTX_BC1_GROUP the
+ * one-block-per-lane encoder (every setting in plain launches; batches keep the
+ * exhaustive fit of level 19 on the other), QTX_BC1_GROUP the
  * 16-lanes-per-block encoder.  Both produce identical bytes; this is a speed
  * knob (and lets tests cover both).  Applies to later plain launches and batches
  * and servers created afterwards.  No counterpart in the reference. */
