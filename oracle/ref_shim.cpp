@@ -105,6 +105,26 @@ int qtxr_bc1_params_init(qtx_bc1_params *out, uint32_t level, int32_t color_mode
     });
 }
 
+// `in` replayed through the reference's setters (apply()), read back through its
+// getters: the values EncodeBlock then sees (BC1Encoder.cpp:251-258 clamps).
+int qtxr_bc1_params_apply(const qtx_bc1_params *in, qtx_bc1_params *out) {
+    return guarded([&] {
+        BC1Encoder e(5, static_cast<BC1Encoder::ColorMode>(in->color_mode), make_interp(in->interpolator));
+        apply(e, in);
+        *out = *in;
+        out->error_mode = static_cast<int32_t>(e.GetErrorMode());
+        out->endpoint_mode = static_cast<int32_t>(e.GetEndpointMode());
+        out->two_ls_passes = e.two_ls_passes;
+        out->two_ep_passes = e.two_ep_passes;
+        out->two_cf_passes = e.two_cf_passes;
+        out->exhaustive = e.exhaustive;
+        out->power_iterations = e.GetPowerIterations();
+        out->search_rounds = e.GetSearchRounds();
+        out->orderings4 = e.GetOrderings4();
+        out->orderings3 = e.GetOrderings3();
+    });
+}
+
 int qtxr_bc1_encode(const qtx_bc1_params *p, const uint8_t *rgba, int32_t w, int32_t h, uint8_t *out) {
     return guarded([&] {
         std::shared_ptr<BC1Encoder> enc;

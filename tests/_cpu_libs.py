@@ -4,6 +4,10 @@
 * ``reference()``: oracle/_ref/libqtxref.so, the reference's own C++ compiled
   from its sources (built only where the reference tree exists; the built
   library travels with the repository snapshot).  None when absent.
+* ``reference_zi()``: oracle/_ref/libqtxref_zi.so, the same sources built with
+  every automatic variable zero-initialised, which makes the reference's one
+  uninitialised read (EndpointMode::LeastSquares, BC1Encoder.cpp:510-515)
+  deterministic.  None when absent.
 """
 import ctypes
 import os
@@ -43,6 +47,14 @@ class _Codec:
         self._call('bc1_params_init', ctypes.byref(p), ctypes.c_uint32(level), ctypes.c_int32(color_mode),
                    ctypes.c_int32(interpolator))
         return p
+
+    def apply(self, raw):
+        """The reference only: raw settings through its setters, read back."""
+        out = BC1Params()
+        fn = getattr(self.lib, self.p + 'bc1_params_apply')
+        fn.argtypes = [ctypes.POINTER(BC1Params), ctypes.POINTER(BC1Params)]
+        self._call('bc1_params_apply', ctypes.byref(raw), ctypes.byref(out))
+        return out
 
     @staticmethod
     def _nblocks(w, h):
@@ -111,6 +123,7 @@ class _Codec:
 
 _oracle = None
 _ref = None
+_ref_zi = None
 
 
 def oracle_path():
@@ -119,6 +132,10 @@ def oracle_path():
 
 def reference_path():
     return os.path.join(ROOT, 'oracle', '_ref', 'libqtxref.so')
+
+
+def reference_zi_path():
+    return os.path.join(ROOT, 'oracle', '_ref', 'libqtxref_zi.so')
 
 
 def oracle():
@@ -134,6 +151,23 @@ def reference():
     if _ref is None and os.path.exists(reference_path()):
         _ref = _Codec(reference_path(), 'qtxr_')
     return _ref
+
+
+def reference_zi():
+    global _ref_zi
+    if _ref_zi is None and os.path.exists(reference_zi_path()):
+        _ref_zi = _Codec(reference_zi_path(), 'qtxr_')
+    return _ref_zi
+
+
+def checker_for(params):
+    """The compiled reference that defines the output for these BC1 settings:
+    the zero-initialising build when the least-squares endpoint mode runs (its
+    g++ build reads an uninitialised accumulator there), else the g++ build.
+    None when that build is absent."""
+    if params.endpoint_mode == 0:
+        return reference_zi()
+    return reference()
 
 
 def op_count(fn):

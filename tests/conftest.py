@@ -28,6 +28,15 @@ def reference():
 
 
 @pytest.fixture(scope='session')
+def reference_zi():
+    from tests._cpu_libs import reference_zi as _r
+    r = _r()
+    if r is None:
+        pytest.skip('zero-initialising reference build (oracle/_ref/libqtxref_zi.so) not built here')
+    return r
+
+
+@pytest.fixture(scope='session')
 def qtx():
     """The product's C ABI.  On a GPU run a missing library is an error, not a skip."""
     from tests._qtx import Qtx

@@ -8,7 +8,13 @@
      (tests/images/__init__.py:18-20, 31-32; test_bc1.py:9-12; test_bc4.py:9-11, 58-70).
 2. ref_vectors.npz -- outputs of the reference C++ compiled from its own sources
    (oracle/_ref/libqtxref.so) on a fixed synthetic input: BC1 at every level,
-   colour mode and interpolator, BC3, BC4, BC5 and every decoder.
+   colour mode and interpolator, BC3, BC4, BC5 and every decoder.  Level 1
+   (EndpointMode::LeastSquares, which reads an uninitialised accumulator in the
+   reference, BC1Encoder.cpp:510-515) comes from the zero-initialising build of
+   the same sources (oracle/_ref/libqtxref_zi.so).
+3. settings_vectors.npz -- the same reference builds on draws from the whole BC1
+   settings space (tests/_settings.py): the settings as the reference's setters
+   leave them and the blocks they give.
 
 Both files are data only (inputs and expected outputs).
 """
@@ -68,10 +74,10 @@ def reference_tests():
 
 def ref_vectors():
     from quicktex_amd import synth
-    from tests._cpu_libs import reference
-    from tests.test_gpu_parity import stress_rgba
-    R = reference()
-    assert R is not None, 'build oracle/_ref first (make -C oracle ref)'
+    from tests._cpu_libs import reference, reference_zi
+    from tests._images import stress_rgba
+    R, Z = reference(), reference_zi()
+    assert R is not None and Z is not None, 'build oracle/_ref first (make -C oracle ref ref-zi)'
     mix = synth.mixture_rgba(64, 16, seed=0x601DE0, near_black=0.15)
     stress = stress_rgba(64, 16, 0x601DE1)
     img = np.concatenate([mix, stress], axis=0)  # 64 x 32 -> 128 blocks
@@ -79,10 +85,8 @@ def ref_vectors():
     for interp in range(4):
         for mode in (4, 3, 0x13):
             for level in range(20):
-                if level == 1 and interp == 2:
-                    continue  # reads uninitialised memory in the reference
                 p = R.params(level, mode, interp)
-                out['bc1_L%d_M%d_I%d' % (level, mode, interp)] = R.bc1_encode(img, p)
+                out['bc1_L%d_M%d_I%d' % (level, mode, interp)] = (Z if level == 1 else R).bc1_encode(img, p)
     rng = np.random.default_rng(0x601DE2)
     a = img.copy()
     a[..., 3] = rng.integers(0, 256, a.shape[:2], dtype=np.uint8)
@@ -113,7 +117,22 @@ def ref_vectors():
     np.savez_compressed(os.path.join(HERE, 'ref_vectors.npz'), **out)
 
 
+def settings_vectors():
+    from tests._cpu_libs import reference, reference_zi
+    from tests.test_settings_space import draws, image
+    R, Z = reference(), reference_zi()
+    img = image()
+    params, blocks = [], []
+    for _, raw in draws():
+        p = R.apply(raw)
+        params.append(p.as_tuple())
+        blocks.append((Z if p.endpoint_mode == 0 else R).bc1_encode(img, p))
+    np.savez_compressed(os.path.join(HERE, 'settings_vectors.npz'), input=img,
+                        params=np.array(params, np.int64), blocks=np.stack(blocks))
+
+
 if __name__ == '__main__':
     reference_tests()
     ref_vectors()
+    settings_vectors()
     print('wrote', os.listdir(HERE))

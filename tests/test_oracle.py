@@ -92,7 +92,7 @@ def test_bc1_ref_vectors(oracle, vec):
         got = oracle.bc1_encode(img, oracle.params(lvl, mode, interp))
         assert (got == vec[key]).all(), key
         n += 1
-    assert n == 20 * 3 * 4 - 3
+    assert n == 20 * 3 * 4  # level 1 from the zero-initialising reference build
 
 
 def test_bc345_ref_vectors(oracle, vec):
