@@ -107,6 +107,20 @@ qtx_status qtx_ctx_set_bc1_encoder(qtx_ctx *ctx, int32_t which);
  * (quicktex/Encoder.h:36-56, quicktex/s3tc/bc1/BC1Encoder.cpp:261-342). */
 qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,
                                  int32_t height, uint8_t *d_blocks, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same encode and bytes as qtx_bc1_encode_device, with
+ * the queued lane kernel's work-queue words and park records in a static array
+ * of the library (8 MiB + 1 KiB per GPU, held from the library's load until the
+ * process ends) instead of the context's lent sets; it calls no allocator.
+ * Textures of at most QTX_NOALLOC_MAX_BLOCKS blocks (ceil(w/4) * ceil(h/4)), e.g.
+ * 2048 x 2048; larger ones return QTX_ERANGE.  Every call on one GPU uses that
+ * GPU's one array, whatever context or stream it comes from: calls whose work
+ * overlaps on a GPU race and give wrong blocks, so the caller serialises them
+ * (one stream, or an event between them). */
+#define QTX_NOALLOC_MAX_BLOCKS 262144
+qtx_status qtx_bc1_encode_device_noalloc(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba,
+                                         int32_t width, int32_t height, uint8_t *d_blocks, void *stream);
 /* BC3Encoder::Encode (quicktex/s3tc/bc3/BC3Encoder.cpp:27-32): BC4 of
  * channel 3 then BC1; params->color_mode must be QTX_COLOR_FOUR. */
 qtx_status qtx_bc3_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba, int32_t width,

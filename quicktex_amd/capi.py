@@ -124,7 +124,7 @@ class Qtx:
         L.qtx_bc3_decode.argtypes = [vp, i32, _U8P, i32, i32, _U8P, sz]
         L.qtx_bc4_decode.argtypes = [vp, u32, _U8P, i32, i32, _U8P, sz]
         L.qtx_bc5_decode.argtypes = [vp, u32, u32, _U8P, i32, i32, _U8P, sz]
-        for n in ('qtx_bc1_encode_device', 'qtx_bc3_encode_device'):
+        for n in ('qtx_bc1_encode_device', 'qtx_bc3_encode_device', 'qtx_bc1_encode_device_noalloc'):
             getattr(L, n).argtypes = [vp, P, vp, i32, i32, vp, vp]
         L.qtx_bc4_encode_device.argtypes = [vp, u32, vp, i32, i32, vp, vp]
         L.qtx_bc5_encode_device.argtypes = [vp, u32, u32, vp, i32, i32, vp, vp]
@@ -252,6 +252,11 @@ class Qtx:
     # -- device buffers (raw pointers, e.g. tensor.data_ptr()) ------------------
     def bc1_encode_device(self, params, d_rgba, w, h, d_blocks, stream=0):
         self.check(self.lib.qtx_bc1_encode_device(self.ctx, ctypes.byref(params), d_rgba, w, h, d_blocks, stream))
+
+    def bc1_encode_device_noalloc(self, params, d_rgba, w, h, d_blocks, stream=0):
+        """qtx_bc1_encode_device without allocation (the library's static scratch)."""
+        self.check(self.lib.qtx_bc1_encode_device_noalloc(self.ctx, ctypes.byref(params), d_rgba, w, h, d_blocks,
+                                                          stream))
 
     def bc3_encode_device(self, params, d_rgba, w, h, d_blocks, stream=0):
         self.check(self.lib.qtx_bc3_encode_device(self.ctx, ctypes.byref(params), d_rgba, w, h, d_blocks, stream))

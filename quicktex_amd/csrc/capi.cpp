@@ -12,6 +12,18 @@
 #include "qtx_internal.h"
 #include "runtime.h"
 
+// Device memory of qtx_bc1_encode_device_noalloc: the queued lane kernel's work
+// queue words (kLaneQueueWords) followed by park records for up to
+// QTX_NOALLOC_MAX_BLOCKS blocks, 8 MiB + 1 KiB in all.  Each device holds its own
+// copy from the library's load until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress gives for the current device (kept in
+// qtx_ctx::d_scratch), never through this name.
+namespace {
+constexpr size_t kScratchQueueBytes = sizeof(uint32_t) * qtx::kLaneQueueWords;
+constexpr size_t kScratchBytes = kScratchQueueBytes + qtx::kParkRecordBytes * (size_t)QTX_NOALLOC_MAX_BLOCKS;
+}  // namespace
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 namespace qtx {
 namespace rt {
 
@@ -232,6 +244,9 @@ qtx_status qtx_ctx_create(int32_t device, qtx_ctx **out) {
     const size_t qbytes = sizeof(uint32_t) * qtx::kLaneQueueWords * qtx_ctx::kQueueSlots;
     if (e == hipSuccess) e = hipMalloc(&ctx->d_queues, qbytes);
     if (e == hipSuccess) e = hipMemset(ctx->d_queues, 0, qbytes);
+    // this device's copy of the static scratch array (the device is current)
+    if (e == hipSuccess) e = hipGetSymbolAddress((void **)&ctx->d_scratch, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && !ctx->d_scratch) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         qtx_ctx_destroy(ctx);
         return hip_fail(e, "context set-up");
@@ -293,6 +308,35 @@ qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
     QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false, ctx->bc1_path,
                                    lease.words, lease.park, ctx->cu_count, (hipStream_t)stream));
     QTX_HIP(lease.commit());
+    return QTX_OK;
+}
+
+qtx_status qtx_bc1_encode_device_noalloc(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *d_rgba,
+                                         int32_t width, int32_t height, uint8_t *d_blocks, void *stream) {
+    qtx_status st = check_ctx(ctx);
+    if (st == QTX_OK) st = check_dims(width, height);
+    if (st != QTX_OK) return st;
+    const uint64_t nblocks = (uint64_t)((width + 3) / 4) * (uint64_t)((height + 3) / 4);
+    if (nblocks > (uint64_t)QTX_NOALLOC_MAX_BLOCKS)
+        return fail(QTX_ERANGE, "texture of %d x %d has %llu blocks; the no-allocation encoder takes at most %d",
+                    width, height, (unsigned long long)nblocks, QTX_NOALLOC_MAX_BLOCKS);
+    qtx::BC1Settings s;
+    int interp = 0;
+    if ((st = to_settings(params, s, interp)) != QTX_OK) return st;
+    if (!d_rgba || !d_blocks) return fail(QTX_EINVAL, "null buffer");
+    if (!ctx->d_scratch) return fail(QTX_EDEVICE, "no scratch memory on device %d", ctx->device);
+    const hipStream_t hs = (hipStream_t)stream;
+    uint32_t *words = nullptr;
+    void *park = nullptr;
+    if (qtx::wants_lane_queue(s, width, height, ctx->bc1_path)) {
+        // the queue and park counters start at zero (the previous call's kernels
+        // leave them so, but a call cut short may not have)
+        words = reinterpret_cast<uint32_t *>(ctx->d_scratch);
+        QTX_HIP(qtx::launch_zero_words(words, (uint32_t)qtx::kLaneQueueWords, hs));
+        if (qtx::bc1_park_bytes(s, width, height, ctx->bc1_path) > 0) park = ctx->d_scratch + kScratchQueueBytes;
+    }
+    QTX_HIP(qtx::launch_bc1_encode(d_rgba, width, height, d_blocks, ctx->d_tables, s, interp, false, ctx->bc1_path,
+                                   words, park, ctx->cu_count, hs));
     return QTX_OK;
 }
 

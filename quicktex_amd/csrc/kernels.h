@@ -89,6 +89,8 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
                              int cu_count, hipStream_t stream);
 hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, int ch0, int ch1, bool bc5,
                              hipStream_t stream);
+// Zeroes n 32-bit words of device memory with a kernel on stream.
+hipError_t launch_zero_words(uint32_t *d_words, uint32_t n, hipStream_t stream);
 // format: QTX_FMT_*; p0/p1: write_alpha for BC1, channels for BC4/BC5
 hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint8_t *d_rgba, int interp, int p0, int p1,
                          hipStream_t stream);

@@ -737,6 +737,12 @@ __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict_
     }
 }
 
+// Writes zero to n words (a kernel, not a memset node, so that a caller's graph
+// capture of the launch replays it faithfully).
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t *__restrict__ words, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) words[i] = 0u;
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -890,6 +896,13 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
         default: return hipErrorInvalidValue;
     }
 #undef QTX_BC1_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(uint32_t *d_words, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(dev::k_zero_words, dim3(std::min<unsigned>(grid_for(n, kThreads), 64u)), dim3(kThreads), 0,
+                       stream, d_words, n);
     return hipGetLastError();
 }
 

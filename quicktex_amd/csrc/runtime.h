@@ -36,6 +36,10 @@ struct qtx_ctx {
     void *d_park[kQueueSlots] = {};
     size_t park_cap[kQueueSlots] = {};
     std::mutex queue_mu;
+    // This device's copy of the library's static scratch array (capi.cpp), the
+    // queue words and park records of qtx_bc1_encode_device_noalloc; looked up at
+    // context creation.
+    uint8_t *d_scratch = nullptr;
 };
 
 namespace qtx {
