@@ -1,6 +1,9 @@
 """The reference's Python API tests (tests/test_bc1.py, test_bc4.py,
-test_texture.py), restated for quicktex_amd.  Blocks and textures are host
-containers and run on the CPU; encoder/decoder tests are marked gpu."""
+test_texture.py), restated and importing the package the way the reference's
+tests do (``from quicktex.s3tc.bc1 import ...``, test_bc1.py:6): the ``quicktex``
+package here is this build's codec under the reference's names.  Blocks and
+textures are host containers and run on the CPU; encoder/decoder tests are
+marked gpu."""
 import json
 import math
 import os
@@ -8,12 +11,12 @@ import os
 import numpy as np
 import pytest
 
-from quicktex_amd import RawTexture
-from quicktex_amd.s3tc.bc1 import BC1Block, BC1Decoder, BC1Encoder, BC1Texture
-from quicktex_amd.s3tc.bc3 import BC3Block, BC3Decoder, BC3Encoder, BC3Texture
-from quicktex_amd.s3tc.bc4 import BC4Block, BC4Decoder, BC4Encoder, BC4Texture
-from quicktex_amd.s3tc.bc5 import BC5Block, BC5Decoder, BC5Encoder, BC5Texture
-from quicktex_amd.s3tc.interpolator import Interpolator, InterpolatorAMD, InterpolatorNvidia, InterpolatorRound
+from quicktex import RawTexture
+from quicktex.s3tc.bc1 import BC1Block, BC1Decoder, BC1Encoder, BC1Texture
+from quicktex.s3tc.bc3 import BC3Block, BC3Decoder, BC3Encoder, BC3Texture
+from quicktex.s3tc.bc4 import BC4Block, BC4Decoder, BC4Encoder, BC4Texture
+from quicktex.s3tc.bc5 import BC5Block, BC5Decoder, BC5Encoder, BC5Texture
+from quicktex.s3tc.interpolator import Interpolator, InterpolatorAMD, InterpolatorNvidia, InterpolatorRound
 
 KAT = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'reference_tests.json')))
 
@@ -236,3 +239,25 @@ def test_round_trips_match_oracle(oracle):
     b5 = BC5Encoder(1, 3).encode(raw)
     assert b5.tobytes() == oracle.bc5_encode(img, 1, 3).tobytes()
     assert BC5Decoder(1, 3).decode(b5).tobytes() == oracle.bc5_decode(np.frombuffer(b5.tobytes(), np.uint8), 37, 29, 1, 3).tobytes()
+
+
+def test_quicktex_names_are_this_codec():
+    """``quicktex`` and ``quicktex_amd`` expose the same objects, so code written
+    against the reference's module names runs this codec unchanged."""
+    import quicktex
+    import quicktex.dds
+    import quicktex.image_utils
+    import quicktex.s3tc.bc1
+    import quicktex_amd
+    import quicktex_amd.dds
+    import quicktex_amd.s3tc.bc1
+    assert quicktex.RawTexture is quicktex_amd.RawTexture and quicktex.Texture is quicktex_amd.Texture
+    assert quicktex.__version__ == quicktex_amd.__version__
+    assert quicktex.s3tc.bc1.BC1Encoder is quicktex_amd.s3tc.bc1.BC1Encoder is BC1Encoder
+    assert quicktex.dds.read is quicktex_amd.dds.read and quicktex.dds.DDSFile is quicktex_amd.dds.DDSFile
+    assert quicktex.image_utils.mip_sizes((8, 2)) == [(8, 2), (4, 1), (2, 1), (1, 1)]
+    for mod in ('bc1', 'bc3', 'bc4', 'bc5', 'interpolator'):
+        import importlib
+        a = importlib.import_module('quicktex.s3tc.' + mod)
+        b = importlib.import_module('quicktex_amd.s3tc.' + mod)
+        assert sorted(a.__all__) == sorted(b.__all__) and all(getattr(a, n) is getattr(b, n) for n in b.__all__)

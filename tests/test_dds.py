@@ -24,7 +24,7 @@ FIXTURES = {
 def dds():
     pytest.importorskip('PIL')
     try:
-        from quicktex_amd import dds as d
+        from quicktex import dds as d
     except ImportError as e:
         pytest.skip('native module not built: %s' % e)
     return d
@@ -64,7 +64,7 @@ def test_save_read_round_trip(dds, name, tmp_path):
 
 
 def test_header_layout(dds, tmp_path):
-    from quicktex_amd.s3tc.bc1 import BC1Texture
+    from quicktex.s3tc.bc1 import BC1Texture
     f = dds.DDSFile()
     f.flags = dds.DDSFlags.TEXTURE | dds.DDSFlags.LINEAR_SIZE | dds.DDSFlags.MIPMAPCOUNT
     f.size = (12, 8)
@@ -89,7 +89,7 @@ def test_header_layout(dds, tmp_path):
 
 
 def test_mip_sizes():
-    from quicktex_amd.image_utils import mip_sizes
+    from quicktex.image_utils import mip_sizes
     assert mip_sizes((63, 63))[:2] == [(63, 63), (31, 31)]  # the docstring's example (image_utils.py:12)
     assert mip_sizes((4096, 4096))[-1] == (1, 1) and len(mip_sizes((4096, 4096))) == 13
     assert mip_sizes((1000, 600)) == [(1000, 600), (500, 300), (250, 150), (125, 75), (62, 37), (31, 18), (15, 9),
@@ -99,3 +99,41 @@ def test_mip_sizes():
     assert mip_sizes((1, 1)) == [(1, 1)]
     with pytest.raises(AssertionError):
         mip_sizes((0, 4))
+
+
+def test_mip_sizes_definition():
+    """Against the chain's definition, level by level: halve with floor, stop at 1,
+    end at the first 1 x 1 level or after mip_count levels."""
+    from quicktex.image_utils import mip_sizes
+    for w in list(range(1, 70)) + [127, 128, 129, 1000, 4095, 4096, 4097, 65535, 65536]:
+        for h in (1, 2, 3, 17, 64, 600, w):
+            for count in (None, 0, 1, 2, 5, 40):
+                chain, size = [], (w, h)
+                while count is None or count == 0 or len(chain) < count:
+                    chain.append(size)
+                    if size == (1, 1):
+                        break
+                    size = (max(size[0] // 2, 1), max(size[1] // 2, 1))
+                assert mip_sizes((w, h), count) == chain, (w, h, count)
+    with pytest.raises(AssertionError):
+        mip_sizes((4, 4), -1)
+
+
+def test_reader_errors(dds, tmp_path):
+    """The reference reader's exceptions: AssertionError for a bad magic, header or
+    pixel-format size and a short file, NotImplementedError for DX10, StopIteration
+    for an unknown FourCC."""
+    src = open(os.path.join(IMAGES, 'bc1/greyscale.dds'), 'rb').read()
+    cases = {
+        'magic': (b'DDX ' + src[4:], AssertionError),
+        'header': (src[:4] + struct.pack('<I', 120) + src[8:], AssertionError),
+        'pixel_format': (src[:76] + struct.pack('<I', 31) + src[80:], AssertionError),
+        'short': (src[:-1], AssertionError),
+        'dx10': (src[:84] + b'DX10' + src[88:], NotImplementedError),
+        'fourcc': (src[:84] + b'DXT3' + src[88:], StopIteration),
+    }
+    for name, (data, exc) in cases.items():
+        path = tmp_path / (name + '.dds')
+        path.write_bytes(data)
+        with pytest.raises(exc):
+            dds.read(path)
