@@ -35,7 +35,8 @@ typedef enum qtx_status {
     QTX_ENOMEM = 3,  /* host or device allocation failed             */
     QTX_EDEVICE = 4, /* HIP runtime error                            */
     QTX_EBUSY = 5,   /* serving queue full / request not finished    */
-    QTX_ENOTSUP = 6  /* request the library cannot serve             */
+    QTX_ENOTSUP = 6, /* request the library cannot serve             */
+    QTX_ECANCELED = 7 /* served request cancelled before it ran       */
 } qtx_status;
 
 /* Interpolator::Type (quicktex/s3tc/interpolator/Interpolator.h:30) */
@@ -327,10 +328,16 @@ typedef struct qtx_server_stats {
     uint64_t max_batch_jobs;   /* largest batch launched                           */
     size_t device_bytes_peak;  /* most request memory reserved at once             */
     double busy_s;             /* time with at least one batch in flight            */
+    uint64_t cancelled;        /* requests cancelled before admission               */
 } qtx_server_stats;
 
 typedef struct qtx_server qtx_server;
-/* Starts the loop on ctx's device (config may be NULL for the defaults). */
+/* Starts the loop on ctx's device (config may be NULL for the defaults).
+ * Batches run on two streams of the server's own, consecutive batches on
+ * alternate ones, so that their kernels overlap; within a batch, decodes and
+ * BC4/BC5 run before BC1/BC3 work.  A batch whose launch the runtime refuses fails
+ * with QTX_EDEVICE and the server goes on; an error raised by work on the device
+ * fails everything in flight and waiting, and later submits return QTX_EDEVICE. */
 qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_server **out);
 /* Queues one request; job->src/dst are host pointers that must stay valid until
  * the request completes; dst_bytes is the capacity of job->dst.  Validation
@@ -342,6 +349,12 @@ qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes
  * result when it is done (the ticket is then released), EBUSY on time-out,
  * EINVAL for an unknown ticket.  Thread-safe. */
 qtx_status qtx_server_wait(qtx_server *s, uint64_t ticket, int32_t timeout_ms, qtx_server_result *result);
+/* Cancels a request that is still waiting for admission: it completes at once
+ * with status QTX_ECANCELED (its buffers are not touched) and its ticket is
+ * collected with qtx_server_wait as any other.  EBUSY if it has been admitted
+ * (it then completes normally), EINVAL for an unknown or finished ticket.
+ * Thread-safe.  No counterpart in the reference, which has no request queue. */
+qtx_status qtx_server_cancel(qtx_server *s, uint64_t ticket);
 qtx_status qtx_server_stats_get(qtx_server *s, qtx_server_stats *stats);
 /* Seconds on the server clock now. */
 double qtx_server_now(qtx_server *s);

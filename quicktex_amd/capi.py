@@ -65,7 +65,7 @@ class ServerStats(ctypes.Structure):
     """qtx_server_stats (include/qtx.h)."""
     _fields_ = [(n, ctypes.c_uint64) for n in ('submitted', 'completed', 'failed', 'batches', 'pixels',
                                                 'max_waiting', 'max_batch_jobs')] + \
-        [('device_bytes_peak', ctypes.c_size_t), ('busy_s', ctypes.c_double)]
+        [('device_bytes_peak', ctypes.c_size_t), ('busy_s', ctypes.c_double), ('cancelled', ctypes.c_uint64)]
 
 
 def job(op, fmt, w, h, src, dst, params=None, chan0=0, chan1=1, write_alpha=False, interpolator=None):
@@ -146,6 +146,7 @@ class Qtx:
         L.qtx_server_submit.argtypes = [vp, JP, sz, ctypes.POINTER(ctypes.c_uint64)]
         L.qtx_server_wait.argtypes = [vp, ctypes.c_uint64, i32, ctypes.POINTER(ServerResult)]
         L.qtx_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.qtx_server_cancel.argtypes = [vp, ctypes.c_uint64]
         L.qtx_server_now.argtypes = [vp]
         L.qtx_server_now.restype = ctypes.c_double
         L.qtx_server_stop.argtypes = [vp]
@@ -348,6 +349,10 @@ class Server:
         self.q.check(self.q.lib.qtx_server_wait(self.h, ticket, timeout_ms, ctypes.byref(r)))
         self._keep.pop(ticket, None)
         return r
+
+    def cancel(self, ticket):
+        """qtx_server_cancel: raises QtxError (status 5, EBUSY) if the request is already running."""
+        self.q.check(self.q.lib.qtx_server_cancel(self.h, ticket))
 
     def stats(self):
         st = ServerStats()

@@ -174,7 +174,7 @@ hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t 
     const BatchJob *dj = d_jobs + g.first;
     if (g.kind == kGroupColour)
         return launch_color_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_queue, g.grid, d_tables, s);
-    if (g.kind == kGroupLane) return launch_lane_batch(g.interp, dj, (int)g.order.size(), g.nunits, d_tables, s);
+    if (g.kind == kGroupLane) return launch_lane_batch(g.interp, dj, (int)g.order.size(), g.nunits, g.grid, d_tables, s);
     return launch_light_batch(dj, (int)g.order.size(), g.nunits, g.grid, s);
 }
 

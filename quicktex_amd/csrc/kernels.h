@@ -104,9 +104,9 @@ constexpr uint32_t kLightUnitBlocks = 256;
 hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, uint32_t *d_queue,
                               unsigned grid, const DeviceTables *d_tables, hipStream_t stream);
 // BC1/BC3 jobs without endpoint search of one interpolator: one block per lane,
-// 256 blocks per unit, one workgroup per unit.
+// 256 blocks per unit; grid workgroups (at most nunits) stride over the units.
 constexpr uint32_t kLaneUnitBlocks = 256;
-hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits,
+hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, unsigned grid,
                              const DeviceTables *d_tables, hipStream_t stream);
 // grid: workgroups (at most nunits; they stride over the units)
 hipError_t launch_light_batch(const BatchJob *d_jobs, int njobs, uint32_t nunits, unsigned grid, hipStream_t stream);

@@ -690,16 +690,19 @@ __device__ __forceinline__ const BatchJob &unit_job(const BatchJob *__restrict__
 }
 
 // BC1/BC3 jobs without endpoint search of one interpolator: one block per lane,
-// one unit of 256 blocks per workgroup.
+// units of 256 blocks; workgroups stride over the units (one unit each when the
+// grid has a workgroup per unit).
 template <int INTERP>
-__global__ __launch_bounds__(256, 4) void k_lane_batch(const BatchJob *__restrict__ jobs, int njobs,
+__global__ __launch_bounds__(256, 4) void k_lane_batch(const BatchJob *__restrict__ jobs, int njobs, uint32_t nunits,
                                                     const DeviceTables *__restrict__ T) {
     lane::load_tables<INTERP, false>(T);
     __syncthreads();
-    const uint32_t u = blockIdx.x;
-    const BatchJob &J = unit_job(jobs, njobs, u);
-    const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
-    if (gid < J.nblocks) encode_lane_block<INTERP>(J.src, J.w, J.h, J.bw, (int)gid, J.dst, T, J.s, J.kind == BJ_ENC_BC3);
+    for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const BatchJob &J = unit_job(jobs, njobs, u);
+        const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
+        if (gid < J.nblocks)
+            encode_lane_block<INTERP>(J.src, J.w, J.h, J.bw, (int)gid, J.dst, T, J.s, J.kind == BJ_ENC_BC3);
+    }
 }
 
 // BC4/BC5 encode and every decoder of a batch: one lane per block, one unit of
@@ -964,25 +967,21 @@ hipError_t launch_color_batch(int interp, const BatchJob *d_jobs, int njobs, uin
     return hipGetLastError();
 }
 
-hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits,
+hipError_t launch_lane_batch(int interp, const BatchJob *d_jobs, int njobs, uint32_t nunits, unsigned grid,
                              const DeviceTables *d_tables, hipStream_t stream) {
-    const dim3 grid(nunits), block(kThreads);
+    const dim3 g(std::max(1u, std::min(grid, nunits))), block(kThreads);
+#define QTX_LB(I)                                                                                                   \
+    case I:                                                                                                         \
+        hipLaunchKernelGGL((dev::k_lane_batch<I>), g, block, kLaneLds, stream, d_jobs, njobs, nunits, d_tables);    \
+        break;
     switch (interp) {
-        case QTX_INTERP_IDEAL:
-            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_IDEAL>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
-            break;
-        case QTX_INTERP_IDEAL_ROUND:
-            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_IDEAL_ROUND>), grid, block, kLaneLds, stream, d_jobs, njobs,
-                               d_tables);
-            break;
-        case QTX_INTERP_NVIDIA:
-            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_NVIDIA>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
-            break;
-        case QTX_INTERP_AMD:
-            hipLaunchKernelGGL((dev::k_lane_batch<QTX_INTERP_AMD>), grid, block, kLaneLds, stream, d_jobs, njobs, d_tables);
-            break;
+        QTX_LB(QTX_INTERP_IDEAL)
+        QTX_LB(QTX_INTERP_IDEAL_ROUND)
+        QTX_LB(QTX_INTERP_NVIDIA)
+        QTX_LB(QTX_INTERP_AMD)
         default: return hipErrorInvalidValue;
     }
+#undef QTX_LB
     return hipGetLastError();
 }
 

@@ -6,20 +6,27 @@
 // and, for a destination in pageable memory, whose output fits in the pinned
 // staging area, stopping at the first that does not fit so that no request is
 // overtaken forever.  The admitted requests form
-// one batch (at most batch_bytes of traffic, so that several batches fit), run
-// as three stages on three streams -- uploads, BC1/BC3 kernels, then the decode
-// kernel and the downloads -- and the loop goes straight back to admitting: up to
-// `slots` batches are on the device at once, so one batch uploads while another
-// computes and a third downloads, and requests that arrive while a batch runs
-// join the next one.  A request whose
-// event has fired is complete and returns its memory; a slot is free again when
-// all of its requests are.
+// one batch (at most batch_bytes of traffic, so that several batches fit).  A
+// batch runs in order on one of two streams -- copies of pageable inputs and the
+// job table, the light kernel (decodes, BC4/BC5), the BC1/BC3 kernels -- and
+// slots alternate between the streams, so the work of consecutive batches
+// overlaps and a small request admitted behind a long level-18 batch does not
+// wait for it (the colour kernels of a served batch leave one workgroup slot per
+// CU free for that).  The loop goes straight back to admitting: up to `slots`
+// batches are on the device at once, and requests that arrive while a batch runs
+// join the next one.  A request whose event has fired is complete and
+// returns its memory; a slot is free again when all of its requests are.
+//
+// A request still waiting can be cancelled.  A batch that fails on an error the
+// runtime raised before running its work fails only its own requests; an error
+// from work on the device ends the server (srv::fault_scope).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -41,6 +48,9 @@ using namespace qtx::rt;
 namespace {
 
 constexpr size_t kAlign = 256;
+// resident workgroups per CU a served batch's colour kernels take (of the four
+// that fit): the rest is room for the next batch's
+constexpr int kServedWgPerCu = bt::kMaxColourWgPerCu - 1;
 constexpr int kQueueWords = 2 * bt::kGroups;  // two words per kernel group
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
@@ -59,6 +69,7 @@ bool is_pinned(const void *p) {
 struct Req {
     uint64_t ticket = 0;
     qtx_job job{};  // host pointers
+    bool admitted = false;
     bt::Prepared prep{};
     bool dst_pinned = false;
     bool src_direct = false;  // the kernel reads the caller's pinned input in place
@@ -72,7 +83,6 @@ struct Req {
 using ReqP = std::shared_ptr<Req>;
 
 struct Slot {
-    hipEvent_t ev_in = nullptr;  // the batch's pageable inputs and job table are on the device
     hipEvent_t ev_k = nullptr;   // its BC1/BC3 kernels are done: their outputs are in host memory
     hipEvent_t ev_l = nullptr;   // its decode / BC4 / BC5 kernel is done
     BatchJob *d_jobs = nullptr;
@@ -93,9 +103,11 @@ struct qtx_server {
     uint8_t *h_stage = nullptr;
     srv::Arena arena, stage;
     std::vector<Slot> slots;
-    // the three stages of every batch: uploads; BC1/BC3 kernels; the decode /
-    // BC4 / BC5 kernel and the encode downloads
-    hipStream_t s_in = nullptr, s_k = nullptr, s_out = nullptr;
+    // the two streams batches run on: slot i's copies and kernels on s_k[i % 2]
+    hipStream_t s_k[2] = {nullptr, nullptr};
+    // test hook: the batch with this number (1 = the first) fails before any of
+    // its work is enqueued, as a call the runtime refuses would (0: never)
+    uint64_t inject_fault_batch = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ReqP> waiting;
@@ -112,6 +124,10 @@ struct qtx_server {
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - origin).count();
     }
+    void sync_streams() const {
+        for (hipStream_t st : s_k)
+            if (st) (void)hipStreamSynchronize(st);
+    }
 };
 
 namespace {
@@ -121,16 +137,17 @@ namespace {
 size_t footprint(const Req &r) { return r.src_direct ? 0 : align_up(r.prep.src_bytes); }
 
 // Enqueue one batch (called without the lock; touches only the slot, the
-// requests, which the loop owns while they are admitted, and the stage streams,
-// which only the loop uses).  Stages:
-//   s_in:  pageable inputs and the job table
-//   s_k:   the BC1/BC3 kernels (after s_in)
-//   s_out: the decode / BC4 / BC5 kernel (after s_in)
-// Kernels read pinned inputs in place over PCIe and store every output straight
-// into host memory -- the caller's pinned buffer or the staging area -- so a batch
-// costs one table upload and its kernels, whatever its size; PCIe reads and writes
-// of different batches and the copy engine's uploads overlap.
-hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
+// requests, which the loop owns while they are admitted, and the slot's stream,
+// which only the loop uses).  Everything of slot i runs in order on s_k[i % 2]:
+// the copies of pageable inputs and the job table, then the light kernel (decodes,
+// BC4/BC5: short and PCIe-bound, so its requests complete first), then the
+// BC1/BC3 kernels.  Kernels read pinned inputs in place over PCIe and store every
+// output straight into host memory -- the caller's pinned buffer or the staging
+// area -- so a batch costs one table upload and its kernels, whatever its size;
+// the batches of the two streams overlap.
+hipError_t enqueue(qtx_server *s, Slot &sl, int slot_index, uint64_t batch_no, const std::vector<ReqP> &reqs) {
+    if (s->inject_fault_batch && batch_no == s->inject_fault_batch) return hipErrorInvalidValue;
+    const hipStream_t st = s->s_k[slot_index & 1];
     std::vector<bt::Prepared> prep(reqs.size());
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < reqs.size() && e == hipSuccess; i++) {
@@ -141,31 +158,33 @@ hipError_t enqueue(qtx_server *s, Slot &sl, const std::vector<ReqP> &reqs) {
         } else {
             uint8_t *dev = s->d_arena + r.dev_off;
             prep[i].job.src = dev;
-            e = hipMemcpyAsync(dev, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, s->s_in);
+            e = hipMemcpyAsync(dev, r.job.src, r.prep.src_bytes, hipMemcpyHostToDevice, st);
         }
         prep[i].job.dst = r.staged ? s->h_stage + r.stage_off : static_cast<uint8_t *>(r.job.dst);
     }
     if (e != hipSuccess) return e;
     bt::Plan plan;
-    const qtx_batch_tuning tuning{bt::kMaxColourWgPerCu, s->cfg.unit_kops};
+    const qtx_batch_tuning tuning{kServedWgPerCu, s->cfg.unit_kops};
     if (bt::plan(prep, tuning, s->ctx->cu_count, s->bc1_path, plan) != QTX_OK) return hipErrorInvalidValue;
     std::memcpy(sl.h_jobs, plan.host.data(), plan.host.size() * sizeof(BatchJob));
-    e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s->s_in);
-    if (e == hipSuccess) e = hipEventRecord(sl.ev_in, s->s_in);
+    e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, plan.host.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st);
     sl.pend_k = sl.pend_l = false;
-    for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++) {
-        bt::Plan::Group grp = plan.groups[g];
-        const bool light = grp.kind == bt::kGroupLight;
-        hipStream_t st = light ? s->s_out : s->s_k;
-        bool &pend = light ? sl.pend_l : sl.pend_k;
-        if (!pend) e = hipStreamWaitEvent(st, sl.ev_in, 0);
-        pend = true;
-        // PCIe-bound: leave the CUs to the encoders
-        if (light) grp.grid = std::min<unsigned>(grp.grid, 2u * (unsigned)s->ctx->cu_count);
-        if (e == hipSuccess) e = bt::enqueue_group(grp, sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {  // the light group, then the colour groups
+        for (size_t g = 0; g < plan.groups.size() && e == hipSuccess; g++) {
+            bt::Plan::Group grp = plan.groups[g];
+            const bool light = grp.kind == bt::kGroupLight;
+            if (light != (pass == 0)) continue;
+            (light ? sl.pend_l : sl.pend_k) = true;
+            // PCIe-bound: leave the CUs to the encoders
+            if (light) grp.grid = std::min<unsigned>(grp.grid, 2u * (unsigned)s->ctx->cu_count);
+            // a colour batch keeps one workgroup slot per CU free for the batch on the
+            // other stream, so a request admitted behind a long batch starts at once
+            else grp.grid = std::min<unsigned>(grp.grid, (unsigned)(kServedWgPerCu * s->ctx->cu_count));
+            e = bt::enqueue_group(grp, sl.d_jobs, sl.d_queues + 2 * g, s->ctx->d_tables, st);
+        }
+        if (e == hipSuccess && pass == 0 && sl.pend_l) e = hipEventRecord(sl.ev_l, st);
     }
-    if (e == hipSuccess && sl.pend_k) e = hipEventRecord(sl.ev_k, s->s_k);
-    if (e == hipSuccess && sl.pend_l) e = hipEventRecord(sl.ev_l, s->s_out);
+    if (e == hipSuccess && sl.pend_k) e = hipEventRecord(sl.ev_k, st);
     return e;
 }
 
@@ -214,10 +233,12 @@ void run(qtx_server *s) {
                 std::string msg;
                 if (q != hipSuccess) {
                     msg = std::string("batch failed: ") + hipGetErrorString(q);
-                    s->failed = true;
-                    s->fail_msg = msg;
+                    if (srv::fault_scope((int)q) == srv::FaultScope::Server) {
+                        s->failed = true;
+                        s->fail_msg = msg;
+                    }
                     lk.unlock();
-                    for (hipStream_t st : {s->s_in, s->s_k, s->s_out}) (void)hipStreamSynchronize(st);
+                    s->sync_streams();
                     lk.lock();
                 }
                 for (auto &r : sl.reqs) {
@@ -240,12 +261,16 @@ void run(qtx_server *s) {
             if (progressed) s->cv_done.notify_all();
         }
         // 2. a new batch on a free slot
+        // a free slot, preferring one whose stream has no batch in flight
         Slot *free_slot = nullptr;
-        for (Slot &sl : s->slots)
-            if (!sl.busy) {
+        int stream_busy[2] = {0, 0};
+        for (size_t i = 0; i < s->slots.size(); i++) stream_busy[i & 1] += s->slots[i].busy ? 1 : 0;
+        for (size_t i = 0; i < s->slots.size(); i++) {
+            Slot &sl = s->slots[i];
+            if (sl.busy) continue;
+            if (!free_slot || (stream_busy[i & 1] == 0 && stream_busy[(free_slot - s->slots.data()) & 1] > 0))
                 free_slot = &sl;
-                break;
-            }
+        }
         if (free_slot && !s->waiting.empty() && !s->failed) {
             std::deque<ReqP> taken;
             size_t batch_host_bytes = 0;
@@ -275,31 +300,38 @@ void run(qtx_server *s) {
                 std::vector<ReqP> reqs(taken.begin(), taken.end());
                 const double t_admit = s->now();
                 for (auto &r : reqs) {
+                    r->admitted = true;
                     r->res.admit_s = t_admit;
                     r->res.batch_jobs = (int32_t)reqs.size();
                 }
                 s->stats.batches++;
+                const uint64_t batch_no = s->stats.batches;
+                const int slot_index = (int)(free_slot - s->slots.data());
                 s->stats.max_batch_jobs = std::max<uint64_t>(s->stats.max_batch_jobs, reqs.size());
                 s->stats.device_bytes_peak = std::max(s->stats.device_bytes_peak, s->arena.used());
                 free_slot->busy = true;
                 update_busy(s, t_admit);
                 lk.unlock();
-                const hipError_t e = enqueue(s, *free_slot, reqs);
+                const hipError_t e = enqueue(s, *free_slot, slot_index, batch_no, reqs);
                 lk.lock();
                 const double t_launch = s->now();
                 if (e != hipSuccess) {
-                    // nothing of this batch can be trusted; wait for what was enqueued
-                    (void)hipStreamSynchronize(s->s_in);
-                    (void)hipStreamSynchronize(s->s_k);
-                    (void)hipStreamSynchronize(s->s_out);
+                    // nothing of this batch can be trusted: wait for what of it was
+                    // enqueued before its memory is reused and its requests fail
+                    lk.unlock();
+                    s->sync_streams();
+                    lk.lock();
                     const std::string msg = std::string("batch enqueue failed: ") + hipGetErrorString(e);
-                    s->failed = true;
-                    s->fail_msg = msg;
+                    if (srv::fault_scope((int)e) == srv::FaultScope::Server) {
+                        s->failed = true;
+                        s->fail_msg = msg;
+                    }
                     for (auto &r : reqs) {
                         release(s, *r);
                         finish(s, *r, QTX_EDEVICE, msg.c_str(), t_launch);
                     }
                     free_slot->busy = false;
+                    free_slot->pend_k = free_slot->pend_l = false;
                     update_busy(s, t_launch);
                     s->cv_done.notify_all();
                 } else {
@@ -327,16 +359,15 @@ void run(qtx_server *s) {
 
 void destroy(qtx_server *s) {
     (void)hipSetDevice(s->ctx->device);
-    for (hipStream_t st : {s->s_in, s->s_k, s->s_out})
-        if (st) (void)hipStreamSynchronize(st);
+    s->sync_streams();
     for (Slot &sl : s->slots) {
-        for (hipEvent_t ev : {sl.ev_in, sl.ev_k, sl.ev_l})
+        for (hipEvent_t ev : {sl.ev_k, sl.ev_l})
             if (ev) (void)hipEventDestroy(ev);
         if (sl.d_jobs) (void)hipFree(sl.d_jobs);
         if (sl.d_queues) (void)hipFree(sl.d_queues);
         if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
     }
-    for (hipStream_t st : {s->s_in, s->s_k, s->s_out})
+    for (hipStream_t st : s->s_k)
         if (st) (void)hipStreamDestroy(st);
     if (s->d_arena) (void)hipFree(s->d_arena);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -372,14 +403,16 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     s->cfg = c;
     s->bc1_path = ctx->bc1_path;
     s->origin = std::chrono::steady_clock::now();
+    if (const char *f = std::getenv("QTX_SERVER_INJECT_FAULT")) s->inject_fault_batch = std::strtoull(f, nullptr, 10);
     hipError_t e = hipMalloc(&s->d_arena, c.device_bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_stage, c.staging_bytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_in, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_out, hipStreamNonBlocking);
+    // two streams: the runtime gives a process a few hardware queues (four by
+    // default) and lets further streams share them, and work on a shared queue runs
+    // in order, so the server keeps to two streams of its own
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k[0], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k[1], hipStreamNonBlocking);
     s->slots.resize((size_t)c.slots);
     for (Slot &sl : s->slots) {
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_k, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_l, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, (size_t)c.max_batch_jobs * sizeof(BatchJob));
@@ -449,6 +482,24 @@ qtx_status qtx_server_wait(qtx_server *s, uint64_t ticket, int32_t timeout_ms, q
     if (!r->done) return fail(QTX_EBUSY, "request %llu not finished", (unsigned long long)ticket);
     *result = r->res;
     s->tickets.erase(ticket);
+    return QTX_OK;
+}
+
+qtx_status qtx_server_cancel(qtx_server *s, uint64_t ticket) {
+    if (!s) return fail(QTX_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(QTX_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+    ReqP r = it->second;
+    if (r->done) return fail(QTX_EINVAL, "request %llu has finished", (unsigned long long)ticket);
+    if (r->admitted || !srv::remove_waiting(s->waiting, r))
+        return fail(QTX_EBUSY, "request %llu is running; it completes normally", (unsigned long long)ticket);
+    r->res.status = QTX_ECANCELED;
+    std::snprintf(r->res.message, sizeof r->res.message, "cancelled before admission");
+    r->res.done_s = s->now();
+    r->done = true;
+    s->stats.cancelled++;
+    s->cv_done.notify_all();
     return QTX_OK;
 }
 

@@ -1,6 +1,7 @@
 // server_core.h -- device-independent parts of the serving loop: the
-// first-fit arena that admission reserves request memory from, and the FIFO
-// admission rule.  Header-only so the host tests compile it without a GPU.
+// first-fit arena that admission reserves request memory from, the FIFO
+// admission rule and the reach of a failed batch.  Header-only so the host tests
+// compile it without a GPU.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -90,6 +91,47 @@ size_t admit_fifo(std::deque<Req> &waiting, size_t max_take, Reserve &&reserve, 
         n++;
     }
     return n;
+}
+
+// How far a HIP error that a batch ran into reaches.  Errors raised by work on the
+// device -- a memory fault, an assert or exception in a kernel, a time-out -- and a
+// lost or uninitialised device leave the device's state unknown: the server fails
+// the batch, everything in flight and everything waiting, and refuses new
+// requests.  Every other error is a call the runtime refused before any of its
+// work ran (an invalid argument, no memory for a launch, a capture conflict): it
+// fails that batch's requests only, and the server goes on with the next batch.
+// Codes are hipError_t values (hip_runtime_api.h).
+enum class FaultScope { Batch, Server };
+inline FaultScope fault_scope(int hip_error) {
+    switch (hip_error) {
+        case 3:    // hipErrorNotInitialized
+        case 4:    // hipErrorDeinitialized
+        case 53:   // hipErrorPriorLaunchFailure
+        case 100:  // hipErrorNoDevice
+        case 101:  // hipErrorInvalidDevice
+        case 201:  // hipErrorInvalidContext
+        case 214:  // hipErrorECCNotCorrectable
+        case 700:  // hipErrorIllegalAddress
+        case 702:  // hipErrorLaunchTimeOut
+        case 709:  // hipErrorContextIsDestroyed
+        case 710:  // hipErrorAssert
+        case 719:  // hipErrorLaunchFailure
+        case 999:  // hipErrorUnknown
+            return FaultScope::Server;
+        default:
+            return FaultScope::Batch;
+    }
+}
+
+// Removes the first element of `waiting` equal to `r` (a request still waiting
+// for admission); true if it was there.
+template <class Req> bool remove_waiting(std::deque<Req> &waiting, const Req &r) {
+    for (auto it = waiting.begin(); it != waiting.end(); ++it)
+        if (*it == r) {
+            waiting.erase(it);
+            return true;
+        }
+    return false;
 }
 
 }  // namespace srv

@@ -62,6 +62,19 @@ int main() {
         CHECK(a.used() == 0);
         CHECK(a.free_runs() == 1 && a.largest_free() == a.capacity());  // fully coalesced
     }
+    // reach of a failed batch: device-side faults and a lost device end the server,
+    // refused calls only their batch (hipError_t values)
+    {
+        using qtx::srv::FaultScope;
+        using qtx::srv::fault_scope;
+        for (int code : {3, 4, 53, 100, 101, 201, 214, 700, 702, 709, 710, 719, 999})
+            CHECK(fault_scope(code) == FaultScope::Server);
+        for (int code : {1, 2, 9, 13, 17, 98, 400, 401, 701, 720, 801, 900, 901})
+            CHECK(fault_scope(code) == FaultScope::Batch);
+        std::deque<int> w = {5, 7, 9, 7};
+        CHECK(qtx::srv::remove_waiting(w, 7) && w.size() == 3 && w[0] == 5 && w[1] == 9 && w[2] == 7);
+        CHECK(!qtx::srv::remove_waiting(w, 4) && w.size() == 3);
+    }
     // FIFO admission: stops at the first request that does not fit
     {
         Arena a(256 * 10, 256);

@@ -7,11 +7,12 @@ subset).  Admission under a small device budget must keep every request
 correct while batching them; validation errors come back from submit.
 """
 import threading
+import time
 
 import numpy as np
 import pytest
 
-from quicktex_amd import hip
+from quicktex_amd import hip, synth
 from quicktex_amd.capi import FMT_BC1, OP_ENCODE, QtxError, Server, job
 from tests._cases import Case
 
@@ -135,3 +136,96 @@ def test_server_stop_drains(qtx):
     srv.stop()  # returns once every queued request is done
     for o, a in zip(outs, alone):
         assert (o == a).all()
+
+
+def _bc1_request(qtx, w, h, level, seed):
+    img = synth.mixture_rgba(w, h, seed=seed, near_black=0.1)
+    p = qtx.params(level, 0x13, 0)
+    return img, p, job(OP_ENCODE, FMT_BC1, w, h, 0, 0, params=p)
+
+
+def test_server_cancel_waiting(qtx):
+    """Requests queued behind a long one are cancelled before admission: they end
+    with QTX_ECANCELED and untouched buffers; the rest complete as alone."""
+    big, pbig, jbig = _bc1_request(qtx, 4096, 2048, 18, 0xCA0)
+    smalls = [_bc1_request(qtx, 64, 48, 5, 0xCA1 + i) for i in range(8)]
+    srv = Server(qtx, device_bytes=256 << 20, staging_bytes=64 << 20, slots=1, max_batch_jobs=1)
+    try:
+        outs = [np.full(jbig.width * jbig.height // 2, 0x5A, np.uint8)] + \
+               [np.full(16 * 12 * 8, 0x5A, np.uint8) for _ in smalls]
+        tickets = [srv.submit(jbig, big.reshape(-1), outs[0])]
+        tickets += [srv.submit(j, img.reshape(-1), o) for (img, _, j), o in zip(smalls, outs[1:])]
+        cancelled = set()
+        for k in (8, 6, 4):
+            try:
+                srv.cancel(tickets[k])
+                cancelled.add(k)
+            except QtxError as e:
+                assert e.status == 5, e  # EBUSY: already admitted
+        res = [srv.wait(t, 120000) for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert cancelled, 'no request was still waiting behind a 4096x2048 level-18 encode'
+    for k, r in enumerate(res):
+        if k in cancelled:
+            assert r.status == 7 and b'cancelled' in r.message, (k, r.status)  # QTX_ECANCELED
+            assert (outs[k] == 0x5A).all(), 'a cancelled request wrote its output'
+        else:
+            assert r.status == 0, (k, r.message)
+            img, p = (big, pbig) if k == 0 else smalls[k - 1][:2]
+            assert (outs[k] == qtx.bc1_encode(img, p)).all(), k
+    assert st['cancelled'] == len(cancelled) and st['completed'] == len(tickets) - len(cancelled)
+    with pytest.raises(QtxError):
+        srv2 = Server(qtx, device_bytes=1 << 20, staging_bytes=1 << 20)
+        try:
+            srv2.cancel(999)  # unknown ticket: EINVAL
+        finally:
+            srv2.stop()
+
+
+def test_server_batch_fault_is_contained(qtx, monkeypatch):
+    """A batch whose launch fails on the host (forced: the server's fault hook makes
+    its second batch fail before any work is enqueued) fails its own requests
+    only; the server goes on and later requests are correct."""
+    monkeypatch.setenv('QTX_SERVER_INJECT_FAULT', '2')
+    reqs = [_bc1_request(qtx, 96, 64, 10, 0xFA0 + i) for i in range(4)]
+    srv = Server(qtx, device_bytes=16 << 20, staging_bytes=16 << 20, slots=2, max_batch_jobs=1)
+    monkeypatch.delenv('QTX_SERVER_INJECT_FAULT')
+    try:
+        results = []
+        for img, p, j in reqs:  # one batch at a time: batch k is request k
+            out = np.zeros(24 * 16 * 8, np.uint8)
+            r = srv.wait(srv.submit(j, img.reshape(-1), out), 60000)
+            results.append((r, out, img, p))
+        st = srv.stats()
+    finally:
+        srv.stop()
+    for k, (r, out, img, p) in enumerate(results):
+        if k == 1:
+            assert r.status == 4 and b'batch enqueue failed' in r.message, (r.status, r.message)
+        else:
+            assert r.status == 0, (k, r.message)
+            assert (out == qtx.bc1_encode(img, p)).all(), k
+    assert st['failed'] == 1 and st['completed'] == 3
+
+
+def test_server_small_request_overtakes_long_batch():
+    """With consecutive batches on the server's two streams, a small request admitted
+    while a long level-18 batch runs finishes before it.  Run in a fresh process
+    (tests/_served_overtake.py), as a serving process would hold the device: the
+    runtime maps a process's streams onto a few hardware queues (four by default)
+    and shares them beyond that, and this process holds streams of other tests."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, '-m', 'tests._served_overtake'], cwd=root, capture_output=True, text=True,
+                       timeout=180)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out['status'] == [0, 0] and out['batch_jobs'] == [1, 1], out
+    assert out['exact'] == [True, True], out
+    assert out['small_done_ms'] < out['big_done_ms'], out
