@@ -63,12 +63,30 @@ size_t block_bytes(int32_t w, int32_t h, int bytes_per_block) {
 
 size_t raw_bytes(int32_t w, int32_t h) { return (size_t)w * (size_t)h * 4; }
 
+namespace {
+// A stream's identity for stream-ordered reuse of queue sets (the null stream: ~0).
+uint64_t stream_id(hipStream_t s) {
+    if (!s) return ~0ull;
+    unsigned long long id = 0;
+    if (hipStreamGetId(s, &id) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;  // unknown: never matched
+    }
+    return (uint64_t)id + 1;
+}
+}  // namespace
+
 QueueLease::QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted) : ctx_(ctx), stream_(stream) {
     if (!wanted) return;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
     ctx->queue_mu.lock();
     locked_ = true;
+    sid_ = stream_id(stream);
+    // a set whose last launch has completed; else one whose last launch was on this
+    // stream: stream order runs that launch (and its drain, which leaves the words
+    // zero) before this one, so the set is free by the time this launch starts
+    int same_stream = -1;
     for (int i = 0; i < qtx_ctx::kQueueSlots; i++) {
         hipEvent_t &ev = ctx->queue_done[i];
         if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
@@ -77,10 +95,13 @@ QueueLease::QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted) : ctx_(ctx
         }
         if (hipEventQuery(ev) == hipSuccess) {  // never recorded, or its launch is done
             slot_ = i;
-            words = ctx->d_queues + (size_t)kLaneQueueWords * i;
-            return;
+            break;
         }
+        if (same_stream < 0 && sid_ != 0 && ctx->queue_stream[i] == sid_) same_stream = i;
     }
+    (void)hipGetLastError();  // hipEventQuery's not-ready status
+    if (slot_ < 0) slot_ = same_stream;
+    if (slot_ >= 0) words = ctx->d_queues + (size_t)kLaneQueueWords * slot_;
 }
 
 void QueueLease::reserve_park(size_t bytes) {
@@ -88,11 +109,14 @@ void QueueLease::reserve_park(size_t bytes) {
     if (slot_ < 0 || bytes == 0) return;
     void *&buf = ctx_->d_park[slot_];
     size_t &cap = ctx_->park_cap[slot_];
-    if (cap < bytes) {  // the set's last launch has completed: its buffer is free
-        if (buf) (void)hipFree(buf);
+    if (cap < bytes) {
+        // stream-ordered: the old buffer is released after the work queued on this
+        // stream so far (the set's last launch completed, or ran on this stream), and
+        // neither call waits for the device
+        if (buf) (void)hipFreeAsync(buf, stream_);
         buf = nullptr;
         cap = 0;
-        if (hipMalloc(&buf, bytes) != hipSuccess) {
+        if (hipMallocAsync(&buf, bytes, stream_) != hipSuccess) {
             buf = nullptr;
             (void)hipGetLastError();
             return;
@@ -104,7 +128,10 @@ void QueueLease::reserve_park(size_t bytes) {
 
 hipError_t QueueLease::commit() {
     hipError_t e = hipSuccess;
-    if (slot_ >= 0) e = hipEventRecord(ctx_->queue_done[slot_], stream_);
+    if (slot_ >= 0) {
+        e = hipEventRecord(ctx_->queue_done[slot_], stream_);
+        ctx_->queue_stream[slot_] = sid_;
+    }
     slot_ = -1;
     if (locked_) ctx_->queue_mu.unlock();
     locked_ = false;
@@ -270,7 +297,8 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
     }
     if (ctx->d_queues) (void)hipFree(ctx->d_queues);
     for (void *p : ctx->d_park)
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QTX_OK;

@@ -31,8 +31,11 @@ struct qtx_ctx {
     static constexpr int kQueueSlots = 64;
     uint32_t *d_queues = nullptr;
     hipEvent_t queue_done[kQueueSlots] = {};  // created on a slot's first use
+    // The stream of each set's last launch (stream_id in capi.cpp): a launch on the
+    // same stream may take the set before that launch completes.
+    uint64_t queue_stream[kQueueSlots] = {};
     // Park buffers of the queued kernel's endpoint searches, one per set, grown on
-    // demand (qtx::bc1_park_bytes).
+    // demand (qtx::bc1_park_bytes) with stream-ordered allocations.
     void *d_park[kQueueSlots] = {};
     size_t park_cap[kQueueSlots] = {};
     std::mutex queue_mu;
@@ -57,11 +60,13 @@ qtx_status to_settings(const qtx_bc1_params *p, BC1Settings &s, int &interp);
 // grow a device buffer to at least need bytes (contents are not kept)
 qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need);
 size_t block_bytes(int32_t w, int32_t h, int bytes_per_block);
-// A set of the context's queue words lent to one launch on `stream`: `words`
-// is nullptr when every set is in use (or the stream is capturing a graph) and
-// the caller launches the queue-less kernel.  commit() marks the set busy until
-// the work enqueued on the stream so far has completed.  Holds the context's
-// queue lock from construction to commit() or destruction.
+// A set of the context's queue words lent to one launch on `stream`: a set
+// whose last launch has completed, else one last lent to a launch on the same
+// stream (stream order makes it free by the time this launch runs); `words` is
+// nullptr when every set is in use by other streams (or the stream is capturing
+// a graph) and the caller launches the queue-less kernel.  commit() marks the set
+// busy until the work enqueued on the stream so far has completed.  Holds the
+// context's queue lock from construction to commit() or destruction.
 class QueueLease {
   public:
     QueueLease(qtx_ctx *ctx, hipStream_t stream, bool wanted);
@@ -77,6 +82,7 @@ class QueueLease {
   private:
     qtx_ctx *ctx_;
     hipStream_t stream_;
+    uint64_t sid_ = 0;
     int slot_ = -1;
     bool locked_ = false;
 };

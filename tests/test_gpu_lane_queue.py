@@ -192,3 +192,27 @@ def test_level19_large_texture_both_encoders(qtx, interp):
     qtx.set_bc1_encoder(0)
     s.destroy()
     assert (outs[0] == outs[2]).all()
+
+
+def test_many_launches_one_stream_reuse_sets(lane, checker):
+    """More launches queued on one stream than the context has queue sets: later
+    launches take sets whose last launch was on the same stream (stream order
+    frees them in time) and keep the queued kernel and its parked searches."""
+    from quicktex_amd import hip
+    q = lane
+    img = synth.mixture_rgba(256, 64, seed=29, near_black=0.1)
+    p = q.params(18, 0x13, 2)
+    want = checker.bc1_encode(img, p)
+    nb = (256 // 4) * (64 // 4)
+    s = hip.Stream()
+    d_in = hip.Buffer.from_array(img)
+    n = 100
+    d_out = hip.Buffer(n * nb * 8)
+    d_out.zero(0xEE)
+    for k in range(n):
+        q.bc1_encode_device(p, d_in.ptr, 256, 64, d_out.ptr + k * nb * 8, s.handle)
+    s.synchronize()
+    got = d_out.download().reshape(n, nb * 8)
+    bad = [k for k in range(n) if not (got[k] == want).all()]
+    assert not bad, 'launches %s differ' % bad[:8]
+    s.destroy()
