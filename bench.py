@@ -74,6 +74,84 @@ def cpu_cores():
     return int(n) if n and n.isdigit() else len(os.sched_getaffinity(0))
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+class ClockSampler:
+    """Samples the GPU's shader clock (amdgpu sysfs: the hwmon freq1_input in Hz, else
+    the starred level of pp_dpm_sclk) every `period` seconds on a thread while running."""
+
+    def __init__(self, period=0.01):
+        import glob
+        import threading
+        self.period, self.samples, self._stop = period, [], threading.Event()
+        self.paths = sorted(glob.glob('/sys/class/drm/card*/device/hwmon/hwmon*/freq1_input'))
+        self.dpm = sorted(glob.glob('/sys/class/drm/card*/device/pp_dpm_sclk'))
+        self._t = threading.Thread(target=self._run, daemon=True)
+
+    def read(self):
+        for p in self.paths:
+            try:
+                v = int(open(p).read().strip())
+                if v > 0:
+                    return v / 1e6
+            except (OSError, ValueError):
+                pass
+        for p in self.dpm:
+            try:
+                for line in open(p):
+                    if line.strip().endswith('*'):
+                        return float(line.split(':')[1].strip().rstrip('*').strip().lower().rstrip('mhz'))
+            except (OSError, ValueError, IndexError):
+                pass
+        return None
+
+    def _run(self):
+        while not self._stop.is_set():
+            v = self.read()
+            if v is not None:
+                self.samples.append((time.perf_counter(), v))
+            self._stop.wait(self.period)
+
+    def __enter__(self):
+        self._t.start()
+        return self
+
+    def __exit__(self, *exc):
+        self._stop.set()
+        self._t.join()
+
+    def summary(self, np):
+        if not self.samples:
+            return None
+        v = np.array([x for _, x in self.samples])
+        half = v[len(v) // 2:]
+        return {'source': 'amdgpu sysfs', 'samples': int(v.size), 'mean_mhz': round(float(v.mean()), 1),
+                'first_mhz': round(float(v[0]), 1), 'last_half_mean_mhz': round(float(half.mean()), 1),
+                'min_mhz': round(float(v.min()), 1), 'max_mhz': round(float(v.max()), 1)}
+
+
+def op_count_per_block(images, params, encode='bc1_encode'):
+    """Algorithmic ops per block (oracle accounting, single-threaded) over sample images."""
+    import numpy as np
+
+    from tests._cpu_libs import op_count
+    nblocks = sum(((im.shape[1] + 3) // 4) * ((im.shape[0] + 3) // 4) for im in images)
+
+    def run(o):
+        for im in images:
+            getattr(o, encode)(np.ascontiguousarray(im), params)
+    counts, weights = op_count(run)
+    return sum(c * w for c, w in zip(counts, weights)) / nblocks
+
+
 def cpu_lib():
     from tests._cpu_libs import oracle, reference
     ref = reference()
@@ -127,20 +205,37 @@ def plain_leg(a, q, hip, np):
     sustained = None
     if a.sustain_steps > 0:
         sev = [hip.Event() for _ in range(a.sustain_steps + 1)]
-        sev[0].record(stream)
-        for s in range(a.sustain_steps):
-            for i in range(4):
-                launch(i)
-            sev[s + 1].record(stream)
-        stream.synchronize()
+        with ClockSampler() as clocks:
+            sev[0].record(stream)
+            for s in range(a.sustain_steps):
+                for i in range(4):
+                    launch(i)
+                sev[s + 1].record(stream)
+            stream.synchronize()
         per_step = np.array([sev[s].elapsed_ms(sev[s + 1]) for s in range(a.sustain_steps)])
         tail = per_step[a.sustain_steps // 2:]
         sustained = {'steps': a.sustain_steps, 'seconds': round(float(per_step.sum()) / 1000.0, 3),
                      'first_step_ms': round(float(per_step[0]), 3),
                      'last_half_ms_per_step': round(float(tail.mean()), 3),
-                     'last_half_mpx_s': round(4 * W * H / (float(tail.mean()) / 1000.0) / 1e6, 1)}
+                     'last_half_mpx_s': round(4 * W * H / (float(tail.mean()) / 1000.0) / 1e6, 1),
+                     'gpu_clock': clocks.summary(np)}
         log('sustained: %d steps in %.2f s, last half %.3f ms per step (%.1f Mpx/s)' % (
             a.sustain_steps, sustained['seconds'], sustained['last_half_ms_per_step'], sustained['last_half_mpx_s']))
+
+    # the drop-in path: host RawTexture bytes (pageable) in, host blocks out, per
+    # interpolator through qtx_bc1_encode (upload, encode, download)
+    host_out = q.bc1_encode(img, params[0])
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for i in range(4):
+            host_out = q.bc1_encode(img, params[i])
+    h2h_s = (time.perf_counter() - t0) / (4 * reps)
+    host_exact = bool(np.array_equal(host_out.reshape(nb, 8), gpu_out[3]))
+    host_to_host = {'mpx_s': round(W * H / h2h_s / 1e6, 1), 'ms_per_encode': round(h2h_s * 1e3, 3),
+                    'input': 'pageable host memory (numpy), as RawTexture holds it',
+                    'equals_device_resident': host_exact}
+    log('host to host: %.2f ms per encode, %.1f Mpx/s' % (h2h_s * 1e3, W * H / h2h_s / 1e6))
 
     # algorithmic ops: oracle accounting on a 16-row sample, scaled per launch
     from tests._cpu_libs import op_count
@@ -182,11 +277,12 @@ def plain_leg(a, q, hip, np):
             exact &= bool((cpu_blocks.reshape(nbp, 8) == gpu_out[i, :nbp]).all())
         cpu_mpx = 4 * W * rows / secs / 1e6
         cpu = {'value': round(cpu_mpx, 3), 'unit': 'Mpixel/s', 'cores': cpu_cores(), 'kind': kind,
+               'cpu_model': cpu_model(),
                'sample': '%d of %d rows (%d blocks) per interpolator, 4 interpolators, %.1f s' % (
                    rows, H, nbp, secs)}
         log('cpu %s: %.3f Mpx/s, bit-exact on sample: %s' % (kind, cpu_mpx, exact))
     stream.destroy()
-    return value, step_ms, per_interp, roofline, cpu, exact, sustained
+    return value, step_ms, per_interp, roofline, cpu, exact, sustained, host_to_host
 
 
 # ---------------------------------------------------------------------------
@@ -257,6 +353,21 @@ def batched_leg(a, q, hip, np):
     batch_bytes = d_batch.download()
     equal = bool(np.array_equal(plain_bytes, batch_bytes))
     b.close()
+    # roofline of the batch: algorithmic ops per block (oracle accounting on the chains
+    # of every 25th texture) x the batch's blocks over the batch's device time
+    nblk = sum((pw // 4) * (ph // 4) for pw, ph in dims)
+    ops_sample = [img for (ti_, _, _, _, img) in tex if ti_ % 25 == 0]
+    ops_block = op_count_per_block(ops_sample, params, 'bc3_encode')
+    batch_tops = ops_block * nblk / (batch_gpu / 1e3) / 1e12
+    roofline = {'kernel': 'k_lane_batch (BC3 level 10: one block per lane, 256 blocks per workgroup) with '
+                          'the batch\'s other group kernels, one graph replay',
+                'bound': 'valu', 'unit': 'TOP/s', 'achieved': round(batch_tops, 3), 'peak': round(VALU_PEAK_TOPS, 2),
+                'frac': round(batch_tops / VALU_PEAK_TOPS, 4), 'algorithmic_ops_per_block': round(ops_block, 1),
+                'blocks': nblk, 'kernel_ms_mean': round(batch_gpu, 3),
+                'ops_sample': 'full chains of every 25th texture (%d blocks)' % sum(
+                    (im.shape[1] // 4) * (im.shape[0] // 4) for im in ops_sample)}
+    log('batched roofline: %.0f ops/block, %.2f TOP/s, %.1f%% of VALU' % (ops_block, batch_tops,
+                                                                        100 * batch_tops / VALU_PEAK_TOPS))
 
     cpu = None
     if not a.no_cpu:
@@ -281,7 +392,7 @@ def batched_leg(a, q, hip, np):
             t = time.perf_counter()
             list(ex.map(lambda k: lib.bc3_encode(tex[k][4], params), sample))
             par = time.perf_counter() - t
-        cpu = {'kind': kind, 'serial_mpx_s': round(spx / serial / 1e6, 3),
+        cpu = {'kind': kind, 'cpu_model': cpu_model(), 'serial_mpx_s': round(spx / serial / 1e6, 3),
                'all_cores_mpx_s': round(spx / par / 1e6, 3), 'cores': threads,
                'sample': '%d mips of every 10th texture, %.2f Mpixel; the reference encodes BC3 on one thread '
                          '(Encoder.h:46,58); all-cores runs the sample mips concurrently' % (len(sample), spx / 1e6),
@@ -301,6 +412,7 @@ def batched_leg(a, q, hip, np):
                    'best_ms': round(tuned_ms, 3)},
         'create_ms': round(create_ms, 2),
         'bit_exact_batched_vs_plain': equal,
+        'roofline': roofline,
         'cpu': cpu,
     }
 
@@ -418,10 +530,13 @@ def served_leg(a, q, hip, np):
         return job(r['op'], r['fmt'], r['w'], r['h'], 0, 0, params=r['params'], chan0=0, chan1=1,
                    interpolator=0)
 
-    def serve(arrivals):
+    def serve(arrivals, pageable=None):
+        """pageable: (sources, destinations) in pageable memory instead of the pinned ones."""
         outbuf.array[:] = 0
-        srv = Server(q, device_bytes=a.serve_device_mib << 20, staging_bytes=64 << 20, slots=3,
-                     max_batch_jobs=1024)
+        srcs = [r['src'] for r in reqs] if pageable is None else pageable[0]
+        dsts = [r['dst'] for r in reqs] if pageable is None else pageable[1]
+        srv = Server(q, device_bytes=a.serve_device_mib << 20, staging_bytes=(64 if pageable is None else 512) << 20,
+                     slots=3, max_batch_jobs=1024)
         try:
             jobs = [job_of(r) for r in reqs]
             base = time.perf_counter()
@@ -430,7 +545,7 @@ def served_leg(a, q, hip, np):
                 if arrivals is not None:
                     while time.perf_counter() - base < arrivals[i]:
                         pass
-                tickets.append(srv.submit(jobs[i], r['src'], r['dst']))
+                tickets.append(srv.submit(jobs[i], srcs[i], dsts[i]))
             res = [srv.wait(t, 600000) for t in tickets]
             stats = srv.stats()
         finally:
@@ -441,13 +556,29 @@ def served_leg(a, q, hip, np):
         done = np.array([x.done_s for x in res])
         span = done.max() - sub.min()
         lat = (done - sub) * 1000.0
-        exact = all(np.array_equal(plain_out[i], reqs[i]['dst']) for i in sample)
+        exact = all(np.array_equal(plain_out[i], dsts[i]) for i in sample)
         return span, lat, stats, exact, np.array([x.batch_jobs for x in res])
 
     sat_span, sat_lat, sat_stats, sat_exact, sat_bj = serve(None)
     sat_rate = N / sat_span
     log('served (all queued at once): %.3f s, %.1f Mpx/s, %d batches' % (sat_span, npx / sat_span / 1e6,
                                                                         sat_stats['batches']))
+    # the same requests from and to pageable memory: inputs are copied into the
+    # device arena, so admission by device footprint takes part
+    pg_master = {k: np.array(v.array) for k, v in masters.items()}
+    pg_blocks = {k: np.array(v.array) for k, v in blocks.items()}
+    pg_src = []
+    for r in reqs:
+        if r['op'] == OP_ENCODE:
+            m = pg_master['bc1' if r['fmt'] == FMT_BC1 else 'bc3' if r['fmt'] == FMT_BC3 else 'bc45']
+            pg_src.append(m[:r['w'] * r['h'] * 4])
+        else:
+            pg_src.append(pg_blocks[r['fmt']][:r['src'].size])
+    pg_dst = [np.zeros(r['out_n'], np.uint8) for r in reqs]
+    pg_span, pg_lat, pg_stats, pg_exact, pg_bj = serve(None, (pg_src, pg_dst))
+    log('served, pageable buffers: %.3f s, %.1f Mpx/s, device memory peak %.1f MiB' % (
+        pg_span, npx / pg_span / 1e6, pg_stats['device_bytes_peak'] / 2**20))
+    del pg_master, pg_blocks, pg_src, pg_dst
     lam = a.serve_rate if a.serve_rate > 0 else 0.6 * sat_rate
     gaps = rng.exponential(1.0 / lam, N)
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -463,6 +594,15 @@ def served_leg(a, q, hip, np):
     for pm in list(masters.values()) + list(blocks.values()) + [outbuf]:
         pm.free()
     pct = lambda x, p: round(float(np.percentile(x, p)), 3)  # noqa: E731
+    # PCIe roofline of the saturated run: every request's input read and output
+    # written across the link (kernels read pinned inputs and write outputs in host
+    # memory), against PCIe 5.0 x16 (64 GB/s each way)
+    host_bytes = sum(r['src'].nbytes + r['out_n'] for r in reqs)
+    pcie_gbs = host_bytes / sat_span / 1e9
+    roofline = {'bound': 'pcie', 'unit': 'GB/s', 'achieved': round(pcie_gbs, 1), 'peak': 128.0,
+                'frac': round(pcie_gbs / 128.0, 4), 'traffic': host_bytes,
+                'traffic_unit': 'host bytes read + written over the whole run',
+                'note': 'peak: PCIe 5.0 x16, 64 GB/s in each direction'}
     return {
         'workload': ('%d requests: BC1/BC3/BC4/BC5 40/30/15/15%%, encode:decode 50:50, sides log-uniform 32..4096, '
                      'levels 0..18 (FourColor, Ideal), %.1f Mpixel; host buffers pinned' % (N, npx / 1e6)),
@@ -476,13 +616,138 @@ def served_leg(a, q, hip, np):
                     'batches': po_stats['batches'], 'mean_batch': round(float(po_bj.mean()), 1),
                     'plain_same_arrivals_p50_ms': pct(plain_lat, 50),
                     'plain_same_arrivals_p99_ms': pct(plain_lat, 99)},
-        'device_bytes_peak': int(max(sat_stats['device_bytes_peak'], po_stats['device_bytes_peak'])),
+        'pageable': {'served_mpx_s': round(npx / pg_span / 1e6, 1), 'span_s': round(pg_span, 3),
+                     'batches': pg_stats['batches'], 'p50_ms': pct(pg_lat, 50), 'p99_ms': pct(pg_lat, 99),
+                     'device_bytes_peak': int(pg_stats['device_bytes_peak']),
+                     'note': 'the same requests all queued at once, sources and destinations in pageable memory: '
+                             'inputs are copied into the device arena (admission by device footprint), outputs '
+                             'staged through pinned memory'},
+        'device_bytes_peak': int(max(sat_stats['device_bytes_peak'], po_stats['device_bytes_peak'],
+                                     pg_stats['device_bytes_peak'])),
         'device_budget_bytes': a.serve_device_mib << 20,
-        'bit_exact_served_vs_plain_sample': bool(sat_exact and po_exact),
+        'roofline': roofline,
+        'bit_exact_served_vs_plain_sample': bool(sat_exact and po_exact and pg_exact),
     }
 
 
 # ---------------------------------------------------------------------------
+def _events_ms(hip, stream, fn, reps):
+    """Mean device time of fn() over reps runs on stream (HIP events), after one untimed run."""
+    fn()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def _median_s(fn, runs=5):
+    fn()  # warm-up (encoder construction and tables are not timed)
+    t = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        t.append(time.perf_counter() - t0)
+    return float(sorted(t)[len(t) // 2])
+
+
+def c0_leg(a, q, hip, np):
+    """The reference's CPU-runnable case: BC1 level 5, Ideal, FourColor, one 256x256
+    texture of the block mixture (alpha 255)."""
+    from quicktex_amd import synth
+    S = 256
+    img = synth.mixture_rgba(S, S, seed=0x5EED0000)
+    p = q.params(5, 4, 0)
+    nb = (S // 4) * (S // 4)
+    d_in, d_out = hip.Buffer.from_array(img), hip.Buffer(nb * 8)
+    stream = hip.Stream()
+    ms = _events_ms(hip, stream, lambda: q.bc1_encode_device(p, d_in.ptr, S, S, d_out.ptr, stream.handle), 200)
+    gpu = d_out.download()
+    host_s = _median_s(lambda: q.bc1_encode(img, p), 20)
+    ops_block = op_count_per_block([img], p)
+    tops = ops_block * nb / (ms / 1e3) / 1e12
+    out = {'workload': 'BC1 level 5 (FourColor, Ideal), 256x256 RGBA8 block mixture, one plain launch',
+           'device_mpx_s': round(S * S / ms / 1e3, 1), 'kernel_ms': round(ms, 4),
+           'host_to_host_mpx_s': round(S * S / host_s / 1e6, 1), 'host_to_host_ms': round(host_s * 1e3, 3),
+           'roofline': {'kernel': 'k_bc1_encode (16 lanes per block; 4096 blocks are below the lane '
+                                  'kernels\' crossover)', 'bound': 'valu', 'unit': 'TOP/s',
+                        'achieved': round(tops, 4), 'peak': round(VALU_PEAK_TOPS, 2),
+                        'frac': round(tops / VALU_PEAK_TOPS, 5), 'algorithmic_ops_per_block': round(ops_block, 1),
+                        'note': 'a launch of 4096 blocks fills under 2% of the GPU\'s lanes; its time is one '
+                                'block\'s latency'}}
+    if not a.no_cpu:
+        lib, kind = cpu_lib()
+        secs = _median_s(lambda: lib.bc1_encode(img, p), 5)
+        exact = bool(np.array_equal(lib.bc1_encode(img, p), gpu))
+        out['cpu_baseline'] = {'value': round(S * S / secs / 1e6, 3), 'unit': 'Mpixel/s', 'cores': cpu_cores(),
+                               'kind': kind, 'cpu_model': cpu_model(),
+                               'sample': 'the whole texture, median of 5 encodes (OpenMP over block rows, as '
+                                         'the reference)', 'bit_exact_vs_gpu': exact}
+        log('c0: GPU %.4f ms (%.0f Mpx/s), host to host %.3f ms, CPU %.2f Mpx/s, exact %s' % (
+            ms, S * S / ms / 1e3, host_s * 1e3, S * S / secs / 1e6, exact))
+    stream.destroy()
+    return out
+
+
+def c1_leg(a, q, hip, np):
+    """BC4 (channel 0) and BC5 (channels 0, 1) encode and decode of a 2048x2048
+    texture of smoothed noise with 20% per-block constants."""
+    from quicktex_amd import synth
+    S = 2048
+    img = synth.smooth_channels(S, S, seed=0x5EED0001, nchan=2)
+    nb = (S // 4) * (S // 4)
+    d_img, d_blk, d_out = hip.Buffer.from_array(img), hip.Buffer(nb * 16), hip.Buffer(S * S * 4)
+    stream = hip.Stream()
+    sh = stream.handle
+    cases = [('bc4_encode', 4.5, lambda: q.bc4_encode_device(0, d_img.ptr, S, S, d_blk.ptr, sh)),
+             ('bc4_decode', 4.5, lambda: q.bc4_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
+             ('bc5_encode', 5.0, lambda: q.bc5_encode_device(0, 1, d_img.ptr, S, S, d_blk.ptr, sh)),
+             ('bc5_decode', 5.0, lambda: q.bc5_decode_device(0, 1, d_blk.ptr, S, S, d_out.ptr, sh))]
+    out = {'workload': '2048x2048, smoothed noise + 20%% per-block constants; BC4 channel 0, BC5 channels 0, 1'}
+    lib, kind = cpu_lib() if not a.no_cpu else (None, None)
+    b4 = b5 = None
+    for name, bpp, fn in cases:
+        ms = _events_ms(hip, stream, fn, 20)
+        gbs = S * S * bpp / (ms / 1e3) / 1e9
+        r = {'kernel_ms': round(ms, 4), 'device_mpx_s': round(S * S / ms / 1e3, 1),
+             'roofline': {'bound': 'hbm', 'unit': 'GB/s', 'achieved': round(gbs, 1), 'peak': 8000.0,
+                          'frac': round(gbs / 8000.0, 4), 'algorithmic_bytes': int(S * S * bpp)}}
+        if lib is not None:
+            if name == 'bc4_encode':
+                cfn = lambda: lib.bc4_encode(img, 0)  # noqa: E731
+            elif name == 'bc5_encode':
+                cfn = lambda: lib.bc5_encode(img, 0, 1)  # noqa: E731
+            elif name == 'bc4_decode':
+                cfn = lambda: lib.bc4_decode(b4, S, S, 0)  # noqa: E731
+            else:
+                cfn = lambda: lib.bc5_decode(b5, S, S, 0, 1)  # noqa: E731
+            if name == 'bc4_decode':
+                d_blk.upload(b4)
+            if name == 'bc5_decode':
+                d_blk.upload(b5)
+            fn()
+            stream.synchronize()
+            got = (d_blk.download(nb * (8 if '4' in name else 16)) if 'encode' in name else d_out.download())
+            secs = _median_s(cfn, 5)
+            want = cfn()
+            if name == 'bc4_encode':
+                b4 = want
+            if name == 'bc5_encode':
+                b5 = want
+            r['cpu_baseline'] = {'value': round(S * S / secs / 1e6, 3), 'unit': 'Mpixel/s', 'cores': 1,
+                                 'kind': kind, 'cpu_model': cpu_model(),
+                                 'sample': 'the whole texture, median of 5 (serial, as the reference runs it)',
+                                 'bit_exact_vs_gpu': bool(np.array_equal(np.asarray(want).reshape(-1), got.reshape(-1)))}
+        out[name] = r
+        log('c1 %s: %.4f ms, %.0f GB/s (%.1f%% of HBM)%s' % (
+            name, ms, gbs, gbs / 80.0, '' if lib is None else ', CPU %.1f Mpx/s, exact %s' % (
+                r['cpu_baseline']['value'], r['cpu_baseline']['bit_exact_vs_gpu'])))
+    stream.destroy()
+    return out
+
+
 def sweep_leg(a, q, hip, np):
     """Plain-kernel throughput per BC1 level (FourColor, Ideal) and of the
     HBM-bound kernels (BC4/BC5 encode, every decoder) on 4096x4096 inputs."""
@@ -665,8 +930,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--legs', default='plain,batched,served,dds',
-                    help='comma list of: plain, batched, served, dds, sweep')
+    ap.add_argument('--legs', default='plain,c0,c1,batched,served,dds',
+                    help='comma list of: plain, c0, c1, batched, served, dds, sweep')
     ap.add_argument('--serve-requests', type=int, default=2000)
     ap.add_argument('--serve-rate', type=float, default=0.0,
                     help='Poisson arrival rate, requests/s (default: 60%% of the measured saturation rate)')
@@ -724,12 +989,16 @@ def main():
                        'texture': '%dx%d' % (W, H), 'level': 18, 'color_mode': 'ThreeColorBlack',
                        'interpolators': list(INTERPS)}}
     if 'plain' in legs:
-        value, step_ms, per_interp, roofline, cpu, exact, sustained = plain_leg(a, q, hip, np)
+        value, step_ms, per_interp, roofline, cpu, exact, sustained, h2h = plain_leg(a, q, hip, np)
         line.update({'value': round(value, 1), 'ms_per_step': round(step_ms, 3),
                      'per_interpolator_mpx_s': per_interp, 'bit_exact_vs_cpu_sample': exact,
-                     'roofline': roofline, 'cpu_baseline': cpu, 'sustained': sustained})
+                     'roofline': roofline, 'cpu_baseline': cpu, 'sustained': sustained, 'host_to_host': h2h})
         if cpu:
             line['gpu_vs_cpu'] = round(value / cpu['value'], 1)
+    if 'c0' in legs:
+        line['c0'] = c0_leg(a, q, hip, np)
+    if 'c1' in legs:
+        line['c1'] = c1_leg(a, q, hip, np)
     if 'batched' in legs:
         line['batched'] = batched_leg(a, q, hip, np)
     if 'served' in legs:

@@ -43,8 +43,15 @@ namespace {
 // (the hardware has it as one instruction, v_dot4_u32_u8) and every other add,
 // multiply, compare, select, conversion, shift or table read as one.  A selector
 // search is priced by what its error needs (the selectors of a trial that is not
-// kept are not work the algorithm requires).  ops = sum(count * weight) is the
-// "algorithmic ops" of a workload.  Counts are per thread: count single-threaded.
+// kept are not work the algorithm requires).  The full selector searches, which
+// dominate the costly levels, are priced at the fewest gfx950 vector
+// instructions that compute them: a palette read as one byte permute per channel
+// and pair from per-channel tables (6), and per texel one dot product and one
+// shift-add per candidate, the maximum over the candidates with 3-input maxima
+// and the running sum with 3-input adds (10 for 4 candidates, 7 for 3), so that
+// no encoder can run them in fewer instructions than they are charged.
+// ops = sum(count * weight) is the "algorithmic ops" of a workload.  Counts are
+// per thread: count single-threaded.
 enum OpKind {
     OP_METRICS, OP_SOLID, OP_EP_GREY, OP_EP_LS, OP_EP_BB, OP_EP_BBINT, OP_EP_PCA, OP_PCA_ITER,
     OP_SEL_FULL4, OP_SEL_FULL3, OP_SEL_CHECK2, OP_SEL_FASTER, OP_SEL_NONE,
@@ -59,8 +66,8 @@ const uint32_t kOpWeight[OP_KINDS] = {
     115,  // bounding box (int)
     360,  // PCA: covariance 16 x 9, scaling, 16 projections x 11
     32,   // one PCA power iteration: 4 rows x (4 mul + 4 add)
-    232,  // 4-candidate full selector search: palette 40 + 16 x (4 dots, 4 mul-subs, 3 max, 1 add)
-    184,  // 3-candidate full selector search: palette 40 + 16 x 9
+    166,  // 4-candidate full selector search: palette 6 + 16 x (4 dots, 4 shift-adds, 2 max3; sum)
+    118,  // 3-candidate full selector search: palette 6 + 16 x (3 dots, 3 shift-adds, 1 max3; sum)
     338,  // Check2 selector search: set-up 50 + 16 x 18
     306,  // Faster selector search: set-up 50 + 16 x 16
     210,  // selector search without error: set-up 50 + 16 x 10
