@@ -918,18 +918,38 @@ struct SearchPark {
 
 // The settings every preset of levels 13-18 shares (BC1Encoder.cpp:74-249): Full
 // error, PCA endpoints, two least-squares and two endpoint passes, no exhaustive
-// cluster fit.  fixed_settings<true> returns s with these as constants.
+// cluster fit.
 __host__ __device__ inline bool hq_settings(const BC1Settings &s) {
     return s.error_mode == QTX_ERROR_FULL && s.endpoint_mode == QTX_EP_PCA && s.ls_passes == 2 &&
            s.ep_passes == 2 && s.exhaustive == 0;
 }
-template <bool HQ> __device__ __forceinline__ BC1Settings fixed_settings(const BC1Settings &s) {
+
+// The settings every preset of levels 0-12 shares (BC1Encoder.cpp:74-249): no
+// endpoint search, one endpoint pass, one cluster-fit pass, no exhaustive fit.
+__host__ __device__ inline bool light_settings(const BC1Settings &s) {
+    return s.search_rounds == 0 && s.ep_passes == 1 && s.cf_passes == 1 && s.exhaustive == 0;
+}
+
+// Encoder builds: for any settings, for those of levels 13-18 (hq_settings) and
+// for those of levels 0-12 (light_settings).  fixed<F> returns s with the
+// settings the build F assumes as constants, so that its encoder carries no
+// branches and no registers for them.
+enum Fixed { kFixNone = 0, kFixHQ = 1, kFixLight = 2 };
+__host__ __device__ inline int fixed_build(const BC1Settings &s) {
+    return hq_settings(s) ? kFixHQ : (light_settings(s) ? kFixLight : kFixNone);
+}
+template <int F> __device__ __forceinline__ BC1Settings fixed(const BC1Settings &s) {
     BC1Settings t = s;
-    if (HQ) {
+    if (F == kFixHQ) {
         t.error_mode = QTX_ERROR_FULL;
         t.endpoint_mode = QTX_EP_PCA;
         t.ls_passes = 2;
         t.ep_passes = 2;
+        t.exhaustive = 0;
+    } else if (F == kFixLight) {
+        t.search_rounds = 0;
+        t.ep_passes = 1;
+        t.cf_passes = 1;
         t.exhaustive = 0;
     }
     return t;

@@ -37,8 +37,9 @@ double bc1_block_ops(const BC1Settings &s);
 // encoder needs enough blocks to fill the GPU, the 16-lane one spreads a block
 // over 16 lanes and wins on few blocks of costly settings.
 constexpr uint64_t kLaneFullBlocks = 262144;  // 256 CUs x 16 waves x 64 lanes
-constexpr uint64_t kLaneMinBlocks = 32768;    // static lane kernel (settings below kLaneQueueMinOps)
-constexpr double kLaneQueueBaseBlocks = 65536.0;  // queued kernel: from this + ops/2 blocks
+constexpr uint64_t kLaneMinBlocks = 32768;    // settings below kLaneCheapOps (levels 0-9)
+constexpr double kLaneCheapOps = 10000.0;
+constexpr double kLaneQueueBaseBlocks = 65536.0;  // costlier settings: from this + ops/2 blocks
 bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path);
 // The same choice for a batch's BC1/BC3 jobs (blocks: the batch's total of such
 // work), whose lane kernel is the static one.
@@ -53,8 +54,10 @@ constexpr int kLaneRetireWord = kLaneQueues * kLaneQueueStride;
 constexpr int kLaneQueueWords = 256;  // per queue, rounded up (1 KiB)
 // The queued kernel serves settings from this estimated cost per block up; on
 // cheaper settings its refills (an atomic and a load per round) outweigh the
-// lanes it saves.
-constexpr double kLaneQueueMinOps = 10000.0;
+// lanes it saves (measured with both kernels' level 0-12 builds: level 10
+// FourColor, 12 k, 0.416 ms static against 0.451 queued; level 11, 16 k, even;
+// level 12, 18 k, 0.595 against 0.559).
+constexpr double kLaneQueueMinOps = 14000.0;
 // Whether launch_bc1_encode would use queue words for this texture (so that the
 // caller need not lend a set otherwise).
 bool wants_lane_queue(const BC1Settings &s, int w, int h, int path);

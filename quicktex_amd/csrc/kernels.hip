@@ -41,24 +41,28 @@ __device__ __forceinline__ uint2 bc4_encode_values(const uint32_t v[16]) {
         mx = max(mx, v[i]);
     }
     if (mx == mn) return make_uint2(mn | (mn << 8), 0u);
-    const int delta = (int)(mx - mn);
+    const uint32_t delta = mx - mn;
     // The reference's level is the number of c in 0..6 with 14 v >= delta (1 + 2c) -
-    // (4 - 14 mn), that is x >= 2 delta c for x = 14 (v - mn) + 4 - delta: so
-    // floor(x / (2 delta)) + 1, at most 7 (x > -2 delta, so x < 0 gives 0).  The
-    // quotient in float: |x| < 2^12 and r is the correctly rounded 1 / (2 delta);
-    // a quotient that is no integer lies at least 1 / 510 below the next one, so
-    // x r + 2^-11 (one rounding) floors to it exactly -- checked for every delta
-    // and every x on the CPU.
-    const float r = 1.0f / (float)(2 * delta);
-    uint64_t bits = 0;
+    // (4 - 14 mn), that is y >= 2 delta (c + 1) for y = 14 (v - mn) + 4 + delta: so
+    // min(floor(y / (2 delta)), 7).  y lies in [5, 3829] and the floor is the high
+    // word of y m with m = ceil(2^32 / (2 delta)): y m / 2^32 exceeds y / (2 delta)
+    // by less than 3829 / 2^32, below the gap 1 / 510 between a quotient that is no
+    // integer and the next one -- checked for every delta and value on the CPU.
+    // The quotient is at most 9; the selector order {1,7,6,5,4,3,2,0} is looked up
+    // by v_perm_b32 from the bytes of the two table words, whose selector values 8
+    // and 9 give the sign byte of a positive byte, 0: level 7's selector.
+    const uint32_t m = 0xFFFFFFFFu / (2u * delta) + 1u;
+    const uint32_t k = 4u + delta - 14u * mn;  // y = 14 v + k (mod 2^32)
+    uint32_t lo = mx | (mn << 8), hi = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int x = 14 * ((int)v[i] - (int)mn) + 4 - delta;
-        const int level = min((int)floorf(fmaf((float)x, r, 0x1p-11f)) + 1, 7);
-        const uint64_t s = (0x02345671u >> (4 * level)) & 7u;  // {1,7,6,5,4,3,2,0}
-        bits |= s << (3 * i);
+        const uint32_t q = __umulhi(14u * v[i] + k, m);
+        const uint32_t sel = __builtin_amdgcn_perm(0x00020304u, 0x05060701u, 0x0C0C0C00u | q);
+        const int b = 16 + 3 * i;  // the 3-bit selector of texel i at bit b of the block
+        if (b < 32) lo |= sel << b;
+        if (b > 29) hi |= b >= 32 ? sel << (b - 32) : sel >> (32 - b);
     }
-    return make_uint2(mx | (mn << 8) | ((uint32_t)(bits & 0xFFFFu) << 16), (uint32_t)(bits >> 16));
+    return make_uint2(lo, hi);
 }
 
 // The eight values of a BC4 block (quicktex/s3tc/bc4/BC4Block.cpp:44-64) as
@@ -183,11 +187,13 @@ __device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rg
     store_lane_block(out, (uint32_t)gid, px, lane::encode_bc1<INTERP, false>(px, s, T), bc3);
 }
 
-// BC1/BC3 encode, one block per lane, for settings without endpoint search.
-template <int INTERP, bool BC3>
+// BC1/BC3 encode, one block per lane (any settings; LIGHT: the settings of
+// levels 0-12 as constants, lane::light_settings).
+template <int INTERP, bool BC3, bool LIGHT>
 __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
-                                                  BC1Settings s) {
+                                                  BC1Settings s_in) {
+    const BC1Settings s = lane::fixed<LIGHT ? lane::kFixLight : lane::kFixNone>(s_in);
     lane::load_tables<INTERP, false>(T);
     __syncthreads();
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -248,16 +254,16 @@ __device__ __forceinline__ void park_search(uint4 *__restrict__ park, uint32_t s
                                     0u);
 }
 
-// HQ: the settings of levels 13-18 as constants (lane::hq_settings), so that the
-// encoder carries no branch on them and no registers for them.
-template <int INTERP, bool BC3, bool HQ>
+// F: the encoder build (lane::Fixed: any settings, those of levels 13-18 or those
+// of levels 0-12 as constants).
+template <int INTERP, bool BC3, int F>
 __global__ __launch_bounds__(lane::kQueueThreads, 1) void k_bc1_lane_queue(const uint8_t *__restrict__ rgba, int w, int h, int bw,
                                                         uint32_t nblocks, uint8_t *__restrict__ out,
                                                         const DeviceTables *__restrict__ T, BC1Settings s_in,
                                                         uint32_t *__restrict__ queue, uint4 *__restrict__ park,
                                                         uint32_t park_stop) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    const BC1Settings s = lane::fixed_settings<HQ>(s_in);
+    const BC1Settings s = lane::fixed<F>(s_in);
     lane::load_tables<INTERP, true>(T);
     // the wave's current range and the number of ranges it found used up live in
     // LDS between refills, so that they hold no registers through the searches
@@ -439,9 +445,12 @@ __device__ __forceinline__ void load_block_texels(const uint8_t *__restrict__ rg
                                                   uint32_t px[16]) {
     const int bx = gid % bw, by = gid / bw;
     if ((w & 3) == 0 && bx * 4 + 3 < w && by * 4 + 3 < h) {
+        // every texel is read once: non-temporal loads keep it out of the caches
+        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
         for (int y = 0; y < 4; y++) {
-            const uint4 row = *reinterpret_cast<const uint4 *>(rgba + 4 * ((size_t)(by * 4 + y) * w + bx * 4));
+            const u32x4 row = __builtin_nontemporal_load(
+                reinterpret_cast<const u32x4 *>(rgba + 4 * ((size_t)(by * 4 + y) * w + bx * 4)));
             px[4 * y + 0] = row.x; px[4 * y + 1] = row.y; px[4 * y + 2] = row.z; px[4 * y + 3] = row.w;
         }
     } else {
@@ -787,7 +796,7 @@ bool use_lane_encoder(const BC1Settings &s, uint64_t blocks, int path) {
     if (p == QTX_BC1_LANE) return true;
     if (blocks >= kLaneFullBlocks) return true;
     const double ops = bc1_block_ops(s);
-    if (ops < kLaneQueueMinOps) return blocks >= kLaneMinBlocks;
+    if (ops < kLaneCheapOps) return blocks >= kLaneMinBlocks;
     return (double)blocks >= kLaneQueueBaseBlocks + 0.5 * ops;
 }
 
@@ -830,21 +839,23 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             const uint32_t stop = drain ? park_stop_env() : 0xFFFFFFFFu;
             uint4 *const park = static_cast<uint4 *>(d_park);
             const dim3 dgrid((unsigned)std::max(cu_count, 1));
-            const bool hq = dev::lane::hq_settings(s);
+            const int fix = dev::lane::fixed_build(s);
 #define QTX_QUEUE_LAUNCH(I, B, H)                                                                                   \
     hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, B, H>), qgrid, qblock, kLaneQueueLds, stream, d_rgba, w, h, bw,    \
                        (uint32_t)nblocks, d_out, d_tables, s, d_queue, park, stop)
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
         if (bc3) {                                                                                                  \
-            if (hq) QTX_QUEUE_LAUNCH(I, true, true);                                                                \
-            else QTX_QUEUE_LAUNCH(I, true, false);                                                                  \
+            if (fix == dev::lane::kFixHQ) QTX_QUEUE_LAUNCH(I, true, dev::lane::kFixHQ);                             \
+            else if (fix == dev::lane::kFixLight) QTX_QUEUE_LAUNCH(I, true, dev::lane::kFixLight);                  \
+            else QTX_QUEUE_LAUNCH(I, true, dev::lane::kFixNone);                                                    \
             if (drain)                                                                                              \
                 hipLaunchKernelGGL((dev::k_bc1_search_drain<I, true>), dgrid, qblock, kDrainLds, stream, d_rgba, w, \
                                    h, bw, d_out, d_tables, s, park, d_queue);                                       \
         } else {                                                                                                    \
-            if (hq) QTX_QUEUE_LAUNCH(I, false, true);                                                               \
-            else QTX_QUEUE_LAUNCH(I, false, false);                                                                 \
+            if (fix == dev::lane::kFixHQ) QTX_QUEUE_LAUNCH(I, false, dev::lane::kFixHQ);                            \
+            else if (fix == dev::lane::kFixLight) QTX_QUEUE_LAUNCH(I, false, dev::lane::kFixLight);                 \
+            else QTX_QUEUE_LAUNCH(I, false, dev::lane::kFixNone);                                                   \
             if (drain)                                                                                              \
                 hipLaunchKernelGGL((dev::k_bc1_search_drain<I, false>), dgrid, qblock, kDrainLds, stream, d_rgba, w,\
                                    h, bw, d_out, d_tables, s, park, d_queue);                                       \
@@ -862,14 +873,19 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             return hipGetLastError();
         }
         const dim3 lgrid(grid_for(nblocks, kThreads));
+        const bool light = dev::lane::light_settings(s);
+#define QTX_LANE_LAUNCH(I, B, L)                                                                                    \
+    hipLaunchKernelGGL((dev::k_bc1_lane<I, B, L>), lgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, nblocks, d_out, \
+                       d_tables, s)
 #define QTX_LANE_CASE(I)                                                                                            \
     case I:                                                                                                         \
-        if (bc3)                                                                                                    \
-            hipLaunchKernelGGL((dev::k_bc1_lane<I, true>), lgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, nblocks, \
-                               d_out, d_tables, s);                                                                 \
-        else                                                                                                        \
-            hipLaunchKernelGGL((dev::k_bc1_lane<I, false>), lgrid, lblock, kLaneLds, stream, d_rgba, w, h, bw, nblocks,\
-                               d_out, d_tables, s);                                                                 \
+        if (bc3) {                                                                                                  \
+            if (light) QTX_LANE_LAUNCH(I, true, true);                                                              \
+            else QTX_LANE_LAUNCH(I, true, false);                                                                   \
+        } else {                                                                                                    \
+            if (light) QTX_LANE_LAUNCH(I, false, true);                                                             \
+            else QTX_LANE_LAUNCH(I, false, false);                                                                  \
+        }                                                                                                           \
         break;
         switch (interp) {
             QTX_LANE_CASE(QTX_INTERP_IDEAL)
@@ -879,6 +895,7 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             default: return hipErrorInvalidValue;
         }
 #undef QTX_LANE_CASE
+#undef QTX_LANE_LAUNCH
         return hipGetLastError();
     }
     const dim3 grid((unsigned)std::min<long long>(((long long)nblocks + 15) / 16, kBc1MaxGrid)), block(kThreads);

@@ -17,7 +17,10 @@ draws cover the rest, in strata chosen to reach every branch of EncodeBlock
   PCA endpoints, two least-squares and two endpoint passes, no exhaustive fit)
   with every other field drawn;
 * ``edges``: search rounds at the exit boundaries 0, 1, 33, 34, 35, 100, 256
-  and orderings at both ends of their range.
+  and orderings at both ends of their range;
+* ``light``: the settings that select the level 0-12 kernel builds (no endpoint
+  search, one endpoint and one cluster-fit pass, no exhaustive fit) with every
+  other field drawn.
 
 Each draw is a raw ``BC1Params``: callers clamp it the way their encoder does
 (the product's qtx_bc1_params_clamp, the reference's setters).
@@ -48,6 +51,8 @@ def _draw(rng, stratum, exhaustive_ok):
         p.error_mode = 0
     elif stratum == 'hq':
         p.error_mode, p.endpoint_mode, p.two_ls_passes, p.two_ep_passes, p.exhaustive = 3, 3, 1, 1, 0
+    elif stratum == 'light':
+        p.search_rounds, p.two_ep_passes, p.two_cf_passes, p.exhaustive = 0, 0, 0, 0
     elif stratum == 'edges':
         p.search_rounds = int(rng.choice(ROUNDS))
         p.orderings4 = int(rng.choice([1, 128]))
@@ -60,7 +65,7 @@ def draws(seed, n_per_stratum, exhaustive_ok=True):
     16 combinations of error mode x endpoint mode with the other fields drawn."""
     rng = np.random.default_rng(seed)
     out = []
-    for stratum in ('free', 'none', 'hq', 'edges'):
+    for stratum in ('free', 'none', 'hq', 'edges', 'light'):
         for k in range(n_per_stratum):
             out.append(('%s%d' % (stratum, k), _draw(rng, stratum, exhaustive_ok)))
     for em in range(4):

@@ -6,7 +6,8 @@ orderings and power iterations at and past their clamps) run through the C ABI
 with the one-block-per-lane encoder forced (its static kernel below 10k
 estimated ops per block, else the queued kernel in its any-settings build, or
 its level 13-18 build for draws that share those settings, with parked
-searches past round 34), the 16-lane encoder forced, and the automatic choice,
+searches past round 34, or its level 0-12 build), the 16-lane encoder forced,
+and the automatic choice,
 each bit-exact against the compiled reference (the zero-initialising build
 where the least-squares endpoint mode runs).  A few draws also run on a
 2048 x 1024 texture, where the automatic choice takes the queued kernel:
