@@ -32,6 +32,14 @@ __device__ __forceinline__ uint32_t load_texel(const uint8_t *__restrict__ rgba,
     return *reinterpret_cast<const uint32_t *>(rgba + 4 * ((size_t)y * (size_t)w + (size_t)x));
 }
 
+// High 32 bits of the product of two 24-bit values (v_mul_hi_u32_u24, a full-rate
+// instruction; the 32-bit v_mul_hi_u32 is not).
+__device__ __forceinline__ uint32_t mul_hi_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // BC4Encoder::EncodeBlock (quicktex/s3tc/bc4/BC4Encoder.cpp:30-71); v[i] in [0,255].
 __device__ __forceinline__ uint2 bc4_encode_values(const uint32_t v[16]) {
     uint32_t mn = 255, mx = 0;
@@ -41,23 +49,35 @@ __device__ __forceinline__ uint2 bc4_encode_values(const uint32_t v[16]) {
         mx = max(mx, v[i]);
     }
     if (mx == mn) return make_uint2(mn | (mn << 8), 0u);
-    const uint32_t delta = mx - mn;
+    const uint32_t delta = mx - mn, d = 2u * delta;
     // The reference's level is the number of c in 0..6 with 14 v >= delta (1 + 2c) -
-    // (4 - 14 mn), that is y >= 2 delta (c + 1) for y = 14 (v - mn) + 4 + delta: so
-    // min(floor(y / (2 delta)), 7).  y lies in [5, 3829] and the floor is the high
-    // word of y m with m = ceil(2^32 / (2 delta)): y m / 2^32 exceeds y / (2 delta)
-    // by less than 3829 / 2^32, below the gap 1 / 510 between a quotient that is no
-    // integer and the next one -- checked for every delta and value on the CPU.
-    // The quotient is at most 9; the selector order {1,7,6,5,4,3,2,0} is looked up
-    // by v_perm_b32 from the bytes of the two table words, whose selector values 8
-    // and 9 give the sign byte of a positive byte, 0: level 7's selector.
-    const uint32_t m = 0xFFFFFFFFu / (2u * delta) + 1u;
-    const uint32_t k = 4u + delta - 14u * mn;  // y = 14 v + k (mod 2^32)
+    // (4 - 14 mn), that is y >= d (c + 1) for y = 14 (v - mn) + 4 + delta and
+    // d = 2 delta: so min(floor(y / d), 7).  This takes q = floor(y / d) + 1 =
+    // floor(y3 / d), y3 = y + d in [7, 4339], as (y3 m) >> 22 with m =
+    // ceil(2^22 / d): y3 m / 2^22 exceeds y3 / d by less than 4339 / 2^22 = 0.00103,
+    // below the gap 1 / 510 between a quotient that is no integer and the next one
+    // -- checked for every delta and value on the CPU.  In 24-bit arithmetic:
+    // y3 2^10 < 2^24 and m <= 2^21, so q is the high word of the 24 x 24-bit product
+    // (y3 2^10) m.  m = floor((2^22 - 1) / d) + 1 from a reciprocal estimate
+    // (within one of the floor) corrected by its remainder.
+    uint32_t f = (uint32_t)((float)0x3FFFFF * __builtin_amdgcn_rcpf((float)d));
+    int r = 0x3FFFFF - (int)(f * d);
+    if (r < 0) {
+        f -= 1u;
+        r += (int)d;
+    }
+    if (r >= (int)d) f += 1u;
+    const uint32_t m = f + 1u;
+    const uint32_t k = (4u + 3u * delta - 14u * mn) << 10;  // y3 2^10 = 14 v 2^10 + k (mod 2^32)
+    // q lies in 1..10.  The selector order {1,7,6,5,4,3,2,0} of levels 0..7 is looked
+    // up by v_perm_b32 from the bytes {0, 1, 7, 6, 5, 4, 3, 2} of the two table words
+    // at index q; indices 8-10 give the sign byte of a positive byte, 0, level 7's
+    // selector, and the result's other bytes index byte 0, 0.
     uint32_t lo = mx | (mn << 8), hi = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint32_t q = __umulhi(14u * v[i] + k, m);
-        const uint32_t sel = __builtin_amdgcn_perm(0x00020304u, 0x05060701u, 0x0C0C0C00u | q);
+        const uint32_t q = mul_hi_u24(__umul24(v[i], 14u << 10) + k, m);
+        const uint32_t sel = __builtin_amdgcn_perm(0x02030405u, 0x06070100u, q);
         const int b = 16 + 3 * i;  // the 3-bit selector of texel i at bit b of the block
         if (b < 32) lo |= sel << b;
         if (b > 29) hi |= b >= 32 ? sel << (b - 32) : sel >> (32 - b);
