@@ -186,28 +186,34 @@ def test_server_cancel_waiting(qtx):
 
 def test_server_batch_fault_is_contained(qtx, monkeypatch):
     """A batch whose launch fails on the host (forced: the server's fault hook makes
-    its second batch fail before any work is enqueued) fails its own requests
-    only; the server goes on and later requests are correct."""
-    monkeypatch.setenv('QTX_SERVER_INJECT_FAULT', '2')
-    reqs = [_bc1_request(qtx, 96, 64, 10, 0xFA0 + i) for i in range(4)]
+    its third batch fail before any of its work is enqueued) fails its own requests
+    only: the requests of the batches in flight beside it and after it are correct,
+    and the server takes new requests afterwards."""
+    monkeypatch.setenv('QTX_SERVER_INJECT_FAULT', '3')
+    reqs = [_bc1_request(qtx, 256, 128, 13, 0xFA0 + i) for i in range(6)]
     srv = Server(qtx, device_bytes=16 << 20, staging_bytes=16 << 20, slots=2, max_batch_jobs=1)
     monkeypatch.delenv('QTX_SERVER_INJECT_FAULT')
     try:
-        results = []
-        for img, p, j in reqs:  # one batch at a time: batch k is request k
-            out = np.zeros(24 * 16 * 8, np.uint8)
-            r = srv.wait(srv.submit(j, img.reshape(-1), out), 60000)
-            results.append((r, out, img, p))
+        # all queued at once: one request per batch in arrival order (batch k is
+        # request k - 1), two batches in flight at a time
+        outs = [np.zeros(64 * 32 * 8, np.uint8) for _ in reqs]
+        tickets = [srv.submit(j, img.reshape(-1), o) for (img, _, j), o in zip(reqs, outs)]
+        results = [srv.wait(t, 60000) for t in tickets]
+        img, p, j = _bc1_request(qtx, 96, 64, 10, 0xFA9)
+        late = np.zeros(24 * 16 * 8, np.uint8)
+        r_late = srv.wait(srv.submit(j, img.reshape(-1), late), 60000)
         st = srv.stats()
     finally:
         srv.stop()
-    for k, (r, out, img, p) in enumerate(results):
-        if k == 1:
+    for k, (r, out, (im, pk, _)) in enumerate(zip(results, outs, reqs)):
+        if k == 2:
             assert r.status == 4 and b'batch enqueue failed' in r.message, (r.status, r.message)
+            assert not out.any(), 'the failed request wrote its output'
         else:
             assert r.status == 0, (k, r.message)
-            assert (out == qtx.bc1_encode(img, p)).all(), k
-    assert st['failed'] == 1 and st['completed'] == 3
+            assert (out == qtx.bc1_encode(im, pk)).all(), k
+    assert r_late.status == 0 and (late == qtx.bc1_encode(img, p)).all()
+    assert st['failed'] == 1 and st['completed'] == 6 and st['batches'] == 7
 
 
 def test_server_small_request_overtakes_long_batch():
