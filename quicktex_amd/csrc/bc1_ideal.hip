@@ -1,0 +1,8 @@
+// bc1_ideal.hip -- the BC1/BC3 encode kernels of the Ideal interpolator.
+#include "bc1_kernels.h"
+
+namespace qtx {
+template hipError_t launch_bc1_interp<QTX_INTERP_IDEAL>(const Bc1Launch &);
+template hipError_t launch_bc1_batch_interp<QTX_INTERP_IDEAL>(bool, const BatchJob *, int, uint32_t, uint32_t *, unsigned,
+                                                 const DeviceTables *, hipStream_t);
+}  // namespace qtx

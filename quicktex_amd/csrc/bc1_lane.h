@@ -236,12 +236,18 @@ __device__ __forceinline__ Metrics metrics(const uint32_t (&px)[16], bool ignore
         s0 += inc ? (int)r : 0; s1 += inc ? (int)g : 0; s2 += inc ? (int)b : 0;
         total += inc;
     }
+    // The flags are taken through an optimisation barrier: left to itself the
+    // compiler rewrites has_black as min(r | g | b) < 4 at its use (the
+    // three-colour-black pass, after the cluster fits), which keeps the 48 channel
+    // values live through the whole encode.
+    uint32_t flags = (grey ? 1u : 0u) | (hb ? 2u : 0u);
+    asm volatile("" : "+v"(flags));
     Metrics m;
     m.mn = rgb(mn0, mn1, mn2);
     m.mx = rgb(mx0, mx1, mx2);
     m.s0 = s0; m.s1 = s1; m.s2 = s2;
-    m.grey = grey;
-    m.has_black = hb;
+    m.grey = (flags & 1u) != 0;
+    m.has_black = (flags & 2u) != 0;
     m.avg = 0;
     if (total > 0) {
         const int h = total / 2;
