@@ -256,10 +256,23 @@ __device__ __forceinline__ Metrics metrics(const uint32_t (&px)[16], bool ignore
     return m;
 }
 
+// The metrics through an optimisation barrier (emits no instruction): what a
+// phase derives from them is computed in that phase, not hoisted into an earlier
+// one and carried (or spilled) in between.
+__device__ __forceinline__ Metrics opaque(const Metrics &in) {
+    Metrics m = in;
+    uint32_t flags = (m.grey ? 1u : 0u) | (m.has_black ? 2u : 0u);
+    asm volatile("" : "+v"(m.mn), "+v"(m.mx), "+v"(m.avg), "+v"(m.s0), "+v"(m.s1), "+v"(m.s2), "+v"(flags));
+    m.grey = (flags & 1u) != 0;
+    m.has_black = (flags & 2u) != 0;
+    return m;
+}
+
 // FindEndpoints (BC1Encoder.cpp:473-677).  Returns low | high << 16.
 template <bool PAL>
-__device__ __forceinline__ uint32_t find_endpoints(uint32_t (&px)[16], const Metrics &m, int endpoint_mode,
+__device__ __forceinline__ uint32_t find_endpoints(uint32_t (&px)[16], const Metrics &m_in, int endpoint_mode,
                                                    bool ignore_black, uint32_t power_iterations) {
+    const Metrics m = opaque(m_in);
     const uint32_t mn0 = chn(m.mn, 0), mn1 = chn(m.mn, 1), mn2 = chn(m.mn, 2);
     const uint32_t mx0 = chn(m.mx, 0), mx1 = chn(m.mx, 1), mx2 = chn(m.mx, 2);
     const int av0 = (int)chn(m.avg, 0), av1 = (int)chn(m.avg, 1), av2 = (int)chn(m.avg, 2);
@@ -625,8 +638,9 @@ __device__ __forceinline__ uint32_t ls_from_pixels(const uint32_t (&px)[16], uin
 
 // RefineBlockLS<M> (BC1Encoder.cpp:852-875)
 template <int INTERP, bool PAL>
-__device__ __forceinline__ Res refine_block_ls(uint32_t (&px)[16], const Blk &bk, const Metrics &m, int M,
+__device__ __forceinline__ Res refine_block_ls(uint32_t (&px)[16], const Blk &bk, const Metrics &m_in, int M,
                                                int error_mode, int passes, Res r) {
+    const Metrics m = opaque(m_in);
     for (int pass = 0; pass < passes; pass++) {
         Res t;
         touch(px);
@@ -675,9 +689,10 @@ __device__ __forceinline__ void sort16(uint32_t (&k)[16], uint32_t (&p)[16]) {
 // Cluster fit, RefineBlockCF<M> (BC1Encoder.cpp:877-928).  Trials are independent
 // (each starts from orig); the first strictly lowest error wins, as in the loop.
 template <int INTERP, bool PAL>
-__device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, const Metrics &m,
+__device__ __forceinline__ Res cluster_fit(uint32_t (&px)[16], const Blk &bk, const Metrics &m_in,
                                            const DeviceTables *__restrict__ T, int M, int error_mode, uint32_t orderings,
                                            bool exhaustive, const Res orig) {
+    const Metrics m = opaque(m_in);
     const bool four = (M & 0xF) == 4;
     // histogram of the input selectors -> starting ordering (Histogram.h:70-79, OrderTable.h:91-104)
     const uint32_t lob = orig.sel & 0x55555555u, hib = (orig.sel >> 1) & 0x55555555u;
