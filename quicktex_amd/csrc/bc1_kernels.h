@@ -84,17 +84,92 @@ __device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rg
     store_lane_block(out, (uint32_t)gid, px, lane::encode_bc1<INTERP, false>(px, s, T), bc3);
 }
 
+// Block lists of the static lane kernel.  A solid block (one RGBA value) costs
+// EncodeBlock a few dozen instructions -- it is written from the single-colour
+// tables (BC1Encoder.cpp:262-268) -- and any other block thousands; with one block
+// per lane, each solid block would idle its lane while the wave encodes the rest.
+// k_bc1_split stores the solid blocks and lists the indices of the others, and
+// k_bc1_lane<..., LIST> encodes the lists, one listed block per lane, so that only
+// a list's last wave runs short.  A split workgroup takes kSplitBlocks blocks and
+// appends its listed ones to list (workgroup % kLaneQueues) with one atomic on that
+// list's count (kLaneQueues counts, kLaneQueueStride words apart, in the launch's
+// queue set: one address takes only so many atomics per second); list q holds
+// list_capacity(nblocks) words from q x that capacity.  The encode's workgroup e
+// takes entries of list e % kLaneQueues (workgroups go to the XCDs round-robin, so
+// a list is split and encoded on one XCD).  kLaneRetireWord counts the encode
+// workgroups that have read their count; the last of them returns the counts and
+// itself to zero, so the next launch on the set (stream-ordered after this one)
+// starts from zero.
+template <int INTERP, bool BC3>
+__global__ __launch_bounds__(256) void k_bc1_split(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
+                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
+                                                   uint32_t *__restrict__ queue, uint32_t *__restrict__ list) {
+    __shared__ uint32_t staged[kSplitBlocks];
+    __shared__ uint32_t nstaged, base;
+    if (threadIdx.x == 0) nstaged = 0;
+    __syncthreads();
+    const uint32_t first = blockIdx.x * kSplitBlocks;
+#pragma unroll 1
+    for (uint32_t j = 0; j < kSplitBlocks; j += 256) {
+        const uint32_t gid = first + j + threadIdx.x;
+        bool listed = false;
+        if (gid < (uint32_t)nblocks) {
+            uint32_t px[16];
+            load_block16(rgba, w, h, bw, (int)gid, px);
+            bool solid = true;
+#pragma unroll
+            for (int i = 1; i < 16; i++) solid &= px[i] == px[0];  // alpha included (ColorBlock.h:80-86)
+            if (solid) store_lane_block(out, gid, px, lane::write_solid_in(T->match[INTERP], px[0]), BC3);
+            else listed = true;
+        }
+        const uint64_t mask = __ballot(listed);
+        uint32_t at = 0;
+        if ((threadIdx.x & 63) == 0 && mask) at = atomicAdd(&nstaged, (uint32_t)__popcll(mask));
+        at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+        if (listed) staged[at + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u))] = gid;
+    }
+    __syncthreads();
+    const uint32_t q = blockIdx.x % kLaneQueues;
+    if (threadIdx.x == 0) base = nstaged ? atomicAdd(&queue[q * kLaneQueueStride], nstaged) : 0u;
+    __syncthreads();
+    uint32_t *const dst = list + (size_t)q * list_capacity((uint32_t)nblocks) + base;
+    for (uint32_t i = threadIdx.x; i < nstaged; i += 256) dst[i] = staged[i];
+}
+
 // BC1/BC3 encode, one block per lane (any settings; LIGHT: the settings of
-// levels 0-12 as constants, lane::light_settings).
-template <int INTERP, bool BC3, bool LIGHT>
+// levels 0-12 as constants, lane::light_settings).  LIST: the blocks k_bc1_split
+// listed (the grid covers every block; workgroups past the list's end return at
+// once).
+template <int INTERP, bool BC3, bool LIGHT, bool LIST>
 __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
-                                                  BC1Settings s_in) {
+                                                  BC1Settings s_in, uint32_t *__restrict__ queue,
+                                                  const uint32_t *__restrict__ list) {
     const BC1Settings s = lane::fixed<LIGHT ? lane::kFixLight : lane::kFixNone>(s_in);
+    uint32_t count = (uint32_t)nblocks, i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t *entries = nullptr;
+    if constexpr (LIST) {
+        __shared__ uint32_t list_count;
+        const uint32_t q = blockIdx.x % kLaneQueues;
+        if (threadIdx.x == 0) {
+            // the acquire orders the read before the retire count: the last
+            // workgroup's reset cannot overtake it
+            list_count = __hip_atomic_load(&queue[q * kLaneQueueStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (atomicAdd(&queue[kLaneRetireWord], 1u) == gridDim.x - 1) {
+                for (int k = 0; k < kLaneQueues; k++) atomicExch(&queue[k * kLaneQueueStride], 0u);
+                atomicExch(&queue[kLaneRetireWord], 0u);
+            }
+        }
+        __syncthreads();
+        count = list_count;
+        const uint32_t first = (blockIdx.x / kLaneQueues) * blockDim.x;
+        if (first >= count) return;  // the whole workgroup
+        i = first + threadIdx.x;
+        entries = list + (size_t)q * list_capacity((uint32_t)nblocks);
+    }
     lane::load_tables<INTERP, false>(T);
     __syncthreads();
-    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (gid < nblocks) encode_lane_block<INTERP>(rgba, w, h, bw, gid, out, T, s, BC3);
+    if (i < count) encode_lane_block<INTERP>(rgba, w, h, bw, LIST ? (int)entries[i] : (int)i, out, T, s, BC3);
 }
 
 // queue[0] hands out units and queue[1] counts the waves that found it empty;
@@ -449,13 +524,22 @@ template <int I, bool B> void launch_queue(const Bc1Launch &a) {
                            a.bw, a.out, a.tables, a.s, a.park, a.queue);
 }
 
+template <int I, bool B, bool LIGHT> void launch_lane_build(const Bc1Launch &a) {
+    if (a.list) {
+        const uint32_t cap = list_capacity((uint32_t)a.nblocks);
+        hipLaunchKernelGGL((dev::k_bc1_split<I, B>), dim3((a.nblocks + kSplitBlocks - 1) / kSplitBlocks), dim3(kThreads),
+                           0, a.stream, a.rgba, a.w, a.h, a.bw, a.nblocks, a.out, a.tables, a.queue, a.list);
+        hipLaunchKernelGGL((dev::k_bc1_lane<I, B, LIGHT, true>), dim3(kLaneQueues * (cap / kThreads)), dim3(kThreads),
+                           kLaneLds, a.stream, a.rgba, a.w, a.h, a.bw, a.nblocks, a.out, a.tables, a.s, a.queue, a.list);
+    } else {
+        hipLaunchKernelGGL((dev::k_bc1_lane<I, B, LIGHT, false>), dim3(a.grid), dim3(kThreads), kLaneLds, a.stream,
+                           a.rgba, a.w, a.h, a.bw, a.nblocks, a.out, a.tables, a.s, nullptr, nullptr);
+    }
+}
+
 template <int I, bool B> void launch_lane(const Bc1Launch &a) {
-    if (a.light)
-        hipLaunchKernelGGL((dev::k_bc1_lane<I, B, true>), dim3(a.grid), dim3(kThreads), kLaneLds, a.stream, a.rgba, a.w,
-                           a.h, a.bw, a.nblocks, a.out, a.tables, a.s);
-    else
-        hipLaunchKernelGGL((dev::k_bc1_lane<I, B, false>), dim3(a.grid), dim3(kThreads), kLaneLds, a.stream, a.rgba,
-                           a.w, a.h, a.bw, a.nblocks, a.out, a.tables, a.s);
+    if (a.light) launch_lane_build<I, B, true>(a);
+    else launch_lane_build<I, B, false>(a);
 }
 
 template <int I, bool B> void launch_group(const Bc1Launch &a) {

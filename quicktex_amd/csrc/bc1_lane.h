@@ -185,25 +185,28 @@ __device__ __forceinline__ uint32_t ep_pair(const uint32_t (&e)[6]) {
 // FindEndpointsSingleColor(result, color, is_3color) (BC1Encoder.cpp:443-456): the pair
 // from the match tables (the 3-colour tables equal the 4-colour ones); err gets the
 // table error.
-template <bool PAL>
-__device__ __forceinline__ uint32_t match_pair(uint32_t color, uint32_t &err) {
-    const uint32_t mr = LL<PAL>().match[0][chn(color, 0)];
-    const uint32_t mg = LL<PAL>().match[1][chn(color, 1)];
-    const uint32_t mb = LL<PAL>().match[0][chn(color, 2)];
+// m: the match tables of the launch's interpolator (5-bit, 6-bit), in LDS or global.
+__device__ __forceinline__ uint32_t match_pair_in(const uint32_t (*m)[256], uint32_t color, uint32_t &err) {
+    const uint32_t mr = m[0][chn(color, 0)];
+    const uint32_t mg = m[1][chn(color, 1)];
+    const uint32_t mb = m[0][chn(color, 2)];
     err = (mr >> 16) + (mg >> 16) + (mb >> 16);
     return ep_make((mr >> 8) & 0xFF, (mg >> 8) & 0xFF, (mb >> 8) & 0xFF) |
            (ep_make(mr & 0xFF, mg & 0xFF, mb & 0xFF) << 16);
 }
-
-// WriteBlockSolid (BC1Encoder.cpp:345-393)
 template <bool PAL>
-__device__ __forceinline__ uint2 write_solid(uint32_t color) {
+__device__ __forceinline__ uint32_t match_pair(uint32_t color, uint32_t &err) {
+    return match_pair_in(LL<PAL>().match, color, err);
+}
+
+// WriteBlockSolid (BC1Encoder.cpp:345-393), from match tables m
+__device__ __forceinline__ uint2 write_solid_in(const uint32_t (*m)[256], uint32_t color) {
     uint32_t mask = 0xAA, min16, max16;
     if ((chn(color, 0) | chn(color, 1) | chn(color, 2)) == 0) {
         min16 = 0; max16 = 1; mask = 0x55;
     } else {
         uint32_t e;
-        const uint32_t ep = match_pair<PAL>(color, e);
+        const uint32_t ep = match_pair_in(m, color, e);
         // the 3-colour attempt uses identical tables, so it never wins (:359-364)
         min16 = ep & 0xFFFFu;
         max16 = ep >> 16;
@@ -216,6 +219,10 @@ __device__ __forceinline__ uint2 write_solid(uint32_t color) {
         }
     }
     return make_uint2(max16 | (min16 << 16), mask * 0x01010101u);
+}
+template <bool PAL>
+__device__ __forceinline__ uint2 write_solid(uint32_t color) {
+    return write_solid_in(LL<PAL>().match, color);
 }
 
 // ColorBlock::GetMetrics (quicktex/ColorBlock.h:88-117)

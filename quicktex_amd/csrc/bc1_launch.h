@@ -16,7 +16,8 @@ namespace qtx {
 // Which BC1/BC3 encode kernel a launch runs.
 enum Bc1Route {
     kRouteGroup = 0,  // k_bc1_encode: one block per group of 16 lanes
-    kRouteLane = 1,   // k_bc1_lane: one block per lane, one fixed block each
+    kRouteLane = 1,   // k_bc1_lane: one block per lane, one fixed block each (with a list:
+                      // k_bc1_split, then k_bc1_lane over the listed non-solid blocks)
     kRouteQueue = 2,  // k_bc1_lane_queue (+ k_bc1_search_drain): blocks dealt from queues
 };
 
@@ -36,6 +37,7 @@ struct Bc1Launch {
     unsigned drain_grid;
     uint32_t *queue;
     uint4 *park;
+    uint32_t *list;  // kRouteLane: the block list (nblocks words), or nullptr
     hipStream_t stream;
 };
 

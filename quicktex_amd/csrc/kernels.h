@@ -30,6 +30,15 @@ inline bool lane_queue_env() {
     }();
     return on;
 }
+// The static lane kernel takes its blocks from a list of the non-solid ones
+// (k_bc1_split) at settings from kLaneListMinOps, unless QTX_LANE_LIST=0.
+inline bool lane_list_env() {
+    static const bool on = [] {
+        const char *e = std::getenv("QTX_LANE_LIST");
+        return !(e && !std::strcmp(e, "0"));
+    }();
+    return on;
+}
 // Estimated ALU ops per BC1 block for the encoder settings.
 double bc1_block_ops(const BC1Settings &s);
 // Whether the lane-per-block encoder serves settings s when `blocks` blocks of
@@ -53,13 +62,30 @@ constexpr int kLaneQueueStride = 16;
 constexpr int kLaneRetireWord = kLaneQueues * kLaneQueueStride;
 constexpr int kLaneQueueWords = 256;  // per queue, rounded up (1 KiB)
 // The queued kernel serves settings from this estimated cost per block up; on
-// cheaper settings its refills (an atomic and a load per round) outweigh the
-// lanes it saves (measured with both kernels' level 0-12 builds: level 10
-// FourColor, 12 k, 0.416 ms static against 0.451 queued; level 11, 16 k, even;
-// level 12, 18 k, 0.595 against 0.559).
-constexpr double kLaneQueueMinOps = 14000.0;
-// Whether launch_bc1_encode would use queue words for this texture (so that the
-// caller need not lend a set otherwise).
+// cheaper settings the static kernel over its block lists is faster (4096x4096
+// block mixture, FourColor: level 11, 16 k, 0.475 ms against 0.547 queued; level
+// 12, 18 k, 0.525 against 0.585; level 13, 34 k, 0.542 against 0.533).
+constexpr double kLaneQueueMinOps = 20000.0;
+// The static kernel lists the non-solid blocks first from this estimated cost per
+// block up; on cheaper settings the listing pass costs more than the idle lanes
+// it saves (same texture: level 7, 4 k, 0.180 ms listed against 0.177 plain;
+// level 8, 6 k, 0.233 against 0.242; level 10, 12 k, 0.376 against 0.420; level
+// 0, 2 k, 0.120 against 0.105).
+constexpr double kLaneListMinOps = 5000.0;
+// The encoder a launch of this texture takes (Bc1Route of bc1_launch.h; the
+// queued route falls back to the static lane kernel when no queue set is lent).
+int bc1_route(const BC1Settings &s, int w, int h, int path);
+// The static lane kernel's block lists (k_bc1_split): a split workgroup takes
+// kSplitBlocks blocks and appends the non-solid ones to one of kLaneQueues lists,
+// each list_capacity(nblocks) words long (the blocks of every kLaneQueues-th
+// split workgroup).
+constexpr uint32_t kSplitBlocks = 1024;
+__host__ __device__ constexpr uint32_t list_capacity(uint32_t nblocks) {
+    return ((nblocks + kSplitBlocks - 1) / kSplitBlocks + kLaneQueues - 1) / kLaneQueues * kSplitBlocks;
+}
+// Whether launch_bc1_encode would use queue words for this texture -- the queued
+// kernel's queues, or the counters of the static lane kernel's block list -- so
+// that the caller need not lend a set otherwise.
 bool wants_lane_queue(const BC1Settings &s, int w, int h, int path);
 // Endpoint searches that go on past round kParkStop are parked by the queued
 // kernel -- their state written to a park buffer, their lane given the next block
@@ -80,13 +106,16 @@ inline uint32_t park_stop_env() {
     }();
     return stop;
 }
-// Bytes of the park buffer a queued launch of this texture needs (0: it parks no
-// search): one record per block at most.
+// Bytes of the buffer lent with the queue words that a launch of this texture
+// needs (0: none): the queued kernel's park records (one per block at most, when
+// its searches can outrun kParkStop), or the static lane kernel's block lists
+// (kLaneQueues x list_capacity words).
 size_t bc1_park_bytes(const BC1Settings &s, int w, int h, int path);
 // d_queue: kLaneQueueWords zeroed words that no other launch uses until this one
-// has completed (the lane encoder then deals blocks from queues), or nullptr.
-// d_park: bc1_park_bytes bytes lent with d_queue (nullptr: searches run to the end
-// where they started).
+// has completed (the lane encoder then deals blocks from queues, or lists the
+// non-solid ones), or nullptr.  d_park: bc1_park_bytes bytes lent with d_queue
+// (nullptr: searches run to the end where they started, and the static lane
+// kernel takes every block).
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
                              const BC1Settings &s, int interp, bool bc3, int path, uint32_t *d_queue, void *d_park,
                              int cu_count, hipStream_t stream);

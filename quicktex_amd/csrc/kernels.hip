@@ -332,15 +332,30 @@ bool use_lane_batch(const BC1Settings &s, uint64_t blocks, int path) {
     return blocks >= kLaneMinBlocks && bc1_block_ops(s) <= kLaneBatchMaxOps;
 }
 
-bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
+int bc1_route(const BC1Settings &s, int w, int h, int path) {
     const uint64_t nblocks = (uint64_t)((w + 3) / 4) * (uint64_t)((h + 3) / 4);
-    return use_lane_encoder(s, nblocks, path) && lane_queue_env() && bc1_block_ops(s) >= kLaneQueueMinOps;
+    if (!use_lane_encoder(s, nblocks, path)) return kRouteGroup;
+    return lane_queue_env() && bc1_block_ops(s) >= kLaneQueueMinOps ? kRouteQueue : kRouteLane;
+}
+
+bool wants_lane_queue(const BC1Settings &s, int w, int h, int path) {
+    const int route = bc1_route(s, w, h, path);
+    return route == kRouteQueue || (route == kRouteLane && lane_list_env() && bc1_block_ops(s) >= kLaneListMinOps);
 }
 
 size_t bc1_park_bytes(const BC1Settings &s, int w, int h, int path) {
-    const uint32_t stop = park_stop_env();
-    if (!wants_lane_queue(s, w, h, path) || stop == 0 || s.search_rounds <= stop) return 0;
-    return kParkRecordBytes * (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4);
+    const size_t nblocks = (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4);
+    switch (bc1_route(s, w, h, path)) {
+        case kRouteQueue: {
+            const uint32_t stop = park_stop_env();
+            return stop == 0 || s.search_rounds <= stop ? 0 : kParkRecordBytes * nblocks;
+        }
+        case kRouteLane:
+            return lane_list_env() && bc1_block_ops(s) >= kLaneListMinOps
+                       ? sizeof(uint32_t) * kLaneQueues * list_capacity((uint32_t)nblocks)
+                       : 0;
+        default: return 0;
+    }
 }
 
 hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out, const DeviceTables *d_tables,
@@ -359,8 +374,9 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
     a.s = s;
     a.bc3 = bc3;
     a.stream = stream;
-    if (use_lane_encoder(s, (uint64_t)nblocks, path)) {
-        if (d_queue && wants_lane_queue(s, w, h, path)) {
+    const int route = bc1_route(s, w, h, path);
+    if (route != kRouteGroup) {
+        if (d_queue && route == kRouteQueue) {
             // persistent: one workgroup of 12 waves per CU, fewer for small launches
             a.route = kRouteQueue;
             a.grid = (unsigned)std::min<long long>(grid_for(nblocks, dev::lane::kQueueThreads), std::max(cu_count, 1));
@@ -375,6 +391,12 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
             a.route = kRouteLane;
             a.grid = grid_for(nblocks, kThreads);
             a.light = dev::lane::light_settings(s);
+            // the block list (solid blocks stored first) when a queue set and list
+            // words came with the launch
+            if (d_queue && d_park && bc1_park_bytes(s, w, h, path) > 0) {
+                a.queue = d_queue;
+                a.list = static_cast<uint32_t *>(d_park);
+            }
         }
     } else {
         a.route = kRouteGroup;
