@@ -7,10 +7,9 @@ blocks (quicktex_amd/synth.py) -- in ThreeColorBlack mode, once per
 interpolator (Ideal, IdealRound, Nvidia, AMD), one plain encode per texture
 (the queued lane kernel, then the drain kernel for the endpoint searches it
 parked).  One step = the four encodes, inputs already resident in HBM.  value =
-Mpixel/s over the timed steps.  sustained: --sustain-steps more steps back to
-back; under continuous load of this VALU-bound kernel the clocks come down after
-a fraction of a second, and the last half of those steps gives the sustained
-rate.
+Mpixel/s over the timed steps, after --warmup steps that bring the shader clock
+up from idle.  sustained: --sustain-steps more steps back to back with the shader
+clock sampled; the last half of those steps gives the sustained rate.
 
 roofline: the encode against the VALU peak (256 CU x 4 SIMD x 16 lanes x
 2.4 GHz = 39.3 T lane-ops/s; the encoder is integer/FP32 ALU work, no HBM or
@@ -62,7 +61,7 @@ INTERPS = ('Ideal', 'IdealRound', 'Nvidia', 'AMD')
 # HBM bytes per BC1 encode on this workload (k_bc1_lane_queue + k_bc1_search_drain),
 # from the committed counter profile (profiles/bc1_l18_encode_4096.md): FETCH_SIZE x 2 +
 # WRITE_SIZE x 1 (copy-kernel calibration), mean over the four interpolators.
-TRAFFIC_BYTES = (644.49e6 + 642.71e6 + 666.80e6 + 649.27e6) / 4
+TRAFFIC_BYTES = (223.15e6 + 218.22e6 + 256.40e6 + 224.12e6) / 4
 
 
 def log(*a):
@@ -200,8 +199,8 @@ def plain_leg(a, q, hip, np):
     log('mean encode per step (ms): ' + ' '.join('%.3f' % m for m in dur.mean(axis=1)))
     gpu_out = d_out.download().reshape(4, nb, 8)
 
-    # sustained load: the same steps back to back for a while longer; under
-    # continuous load the clocks come down after a fraction of a second
+    # sustained load: the same steps back to back for a while longer, the shader
+    # clock sampled meanwhile
     sustained = None
     if a.sustain_steps > 0:
         sev = [hip.Event() for _ in range(a.sustain_steps + 1)]
@@ -924,10 +923,11 @@ def dds_leg(a, q, hip, np):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--sustain-steps', type=int, default=200,
                     help='plain leg: steps run back to back after the timed ones (0: none)')
-    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=10,
+                    help='untimed steps first (the shader clock ramps up from idle over the first ~50 ms)')
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--legs', default='plain,c0,c1,batched,served,dds',

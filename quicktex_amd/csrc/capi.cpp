@@ -12,15 +12,19 @@
 #include "qtx_internal.h"
 #include "runtime.h"
 
-// Device memory of qtx_bc1_encode_device_noalloc: the queued lane kernel's work
-// queue words (kLaneQueueWords) followed by park records for up to
-// QTX_NOALLOC_MAX_BLOCKS blocks, 8 MiB + 1 KiB in all.  Each device holds its own
+// Device memory of qtx_bc1_encode_device_noalloc: the lane kernels' work queue
+// words (kLaneQueueWords) followed by park records for up to
+// QTX_NOALLOC_MAX_BLOCKS blocks (or, for the static lane kernel, its block lists,
+// at most 1 MiB there), 8 MiB + 1 KiB in all.  Each device holds its own
 // copy from the library's load until the process ends; host code reaches it only
 // through the address hipGetSymbolAddress gives for the current device (kept in
 // qtx_ctx::d_scratch), never through this name.
 namespace {
 constexpr size_t kScratchQueueBytes = sizeof(uint32_t) * qtx::kLaneQueueWords;
 constexpr size_t kScratchBytes = kScratchQueueBytes + qtx::kParkRecordBytes * (size_t)QTX_NOALLOC_MAX_BLOCKS;
+static_assert(sizeof(uint32_t) * qtx::kLaneQueues * qtx::list_capacity(QTX_NOALLOC_MAX_BLOCKS) <=
+                  qtx::kParkRecordBytes * (size_t)QTX_NOALLOC_MAX_BLOCKS,
+              "the block lists fit where the park records go");
 }  // namespace
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 

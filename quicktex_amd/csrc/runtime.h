@@ -34,8 +34,9 @@ struct qtx_ctx {
     // The stream of each set's last launch (stream_id in capi.cpp): a launch on the
     // same stream may take the set before that launch completes.
     uint64_t queue_stream[kQueueSlots] = {};
-    // Park buffers of the queued kernel's endpoint searches, one per set, grown on
-    // demand (qtx::bc1_park_bytes) with stream-ordered allocations.
+    // One buffer per set, lent with its words: the queued kernel's parked endpoint
+    // searches, or the static lane kernel's block lists; grown on demand
+    // (qtx::bc1_park_bytes) with stream-ordered allocations.
     void *d_park[kQueueSlots] = {};
     size_t park_cap[kQueueSlots] = {};
     std::mutex queue_mu;
