@@ -53,10 +53,9 @@ __device__ __forceinline__ void bc4_decode_into(uint32_t px[16], uint32_t a0, ui
     }
 }
 
-// The 16 texels of block gid (edge blocks wrap like the reference's texel reads).
-__device__ __forceinline__ void load_block_texels(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid,
+// The 16 texels of block (bx, by) (edge blocks wrap like the reference's texel reads).
+__device__ __forceinline__ void load_block_texels(const uint8_t *__restrict__ rgba, int w, int h, int bx, int by,
                                                   uint32_t px[16]) {
-    const int bx = gid % bw, by = gid / bw;
     if ((w & 3) == 0 && bx * 4 + 3 < w && by * 4 + 3 < h) {
         // every texel is read once: non-temporal loads keep it out of the caches
         typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -74,10 +73,10 @@ __device__ __forceinline__ void load_block_texels(const uint8_t *__restrict__ rg
 
 // BC4 (one channel) or BC5 (two channels) of block gid: one lane per block.
 template <bool BC5>
-__device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rgba, int w, int h, int bw, int gid, int ch0,
-                                                  int ch1, uint8_t *__restrict__ out) {
+__device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rgba, int w, int h, int gid, int bx, int by,
+                                                  int ch0, int ch1, uint8_t *__restrict__ out) {
     uint32_t px[16];
-    load_block_texels(rgba, w, h, bw, gid, px);
+    load_block_texels(rgba, w, h, bx, by, px);
     uint32_t v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = chn(px[i], ch0);
@@ -93,10 +92,13 @@ __device__ __forceinline__ void encode_bc45_block(const uint8_t *__restrict__ rg
 }
 
 template <bool BC5>
-__global__ __launch_bounds__(256) void k_bc4_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
-                                                    int ch0, int ch1, uint8_t *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_bc4_encode(const uint8_t *__restrict__ rgba, int w, int h, int bw, RowDiv rd,
+                                                    int nblocks, int ch0, int ch1, uint8_t *__restrict__ out) {
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (gid < nblocks) encode_bc45_block<BC5>(rgba, w, h, bw, gid, ch0, ch1, out);
+    if (gid >= nblocks) return;
+    int bx, by;
+    block_xy((uint32_t)gid, (uint32_t)bw, rd, bx, by);
+    encode_bc45_block<BC5>(rgba, w, h, gid, bx, by, ch0, ch1, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,8 +129,7 @@ __device__ __forceinline__ void store_row_nt(uint8_t *row, uint32_t a, uint32_t 
 // Decode of block gid of a bw x bh block texture into the w x h RGBA8 texture.
 template <int INTERP, int KIND>
 __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks, int w, int h, int bw, int bh, int p0,
-                                             int p1, uint8_t *__restrict__ rgba, int gid) {
-    const int bx = gid % bw, by = gid / bw;
+                                             int p1, uint8_t *__restrict__ rgba, int gid, int bx, int by) {
     uint32_t px[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = 0xFF000000u;  // Color() = (0, 0, 0, 255)
@@ -157,9 +158,10 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
         const uint32_t bc1 = (three ? pal[2] : pal[3]) | 0xFF000000u;
         const uint32_t bc2 = pal[1] | 0xFF000000u;
         const uint32_t bc3 = three ? (write_alpha ? 0u : 0xFF000000u) : (pal[2] | 0xFF000000u);
-        if (KIND == DEC_BC1) {
+        {
             // byte planes: R holds the red of colours 0-3 in bytes 0-3, and so on; a
             // texel is then two v_perm_b32 (red and green, blue and alpha) of its selector
+            // (BC3: alpha from the alpha block below)
             const uint32_t lo01 = __builtin_amdgcn_perm(bc1, bc0, 0x05010400u);
             const uint32_t hi01 = __builtin_amdgcn_perm(bc1, bc0, 0x07030602u);
             const uint32_t lo23 = __builtin_amdgcn_perm(bc3, bc2, 0x05010400u);
@@ -173,12 +175,6 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
                 const uint32_t s = (cb.y >> (2 * i)) & 3u;
                 const uint32_t sel = s * 0x101u + 0x0C0C0400u;  // bytes: s, s + 4, zero, zero
                 px[i] = __builtin_amdgcn_perm(G, R, sel) | __builtin_amdgcn_perm(A, B, (sel << 16) | 0x0C0Cu);
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t s = (cb.y >> (2 * i)) & 3u;
-                px[i] = s == 0 ? bc0 : (s == 1 ? bc1 : (s == 2 ? bc2 : bc3));
             }
         }
         if (KIND == DEC_BC3)
@@ -228,9 +224,12 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
 
 template <int INTERP, int KIND>
 __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ blocks, int w, int h, int bw, int bh,
-                                                int p0, int p1, uint8_t *__restrict__ rgba) {
+                                                RowDiv rd, int p0, int p1, uint8_t *__restrict__ rgba) {
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (gid < bw * bh) decode_block<INTERP, KIND>(blocks, w, h, bw, bh, p0, p1, rgba, gid);
+    if (gid >= bw * bh) return;
+    int bx, by;
+    block_xy((uint32_t)gid, (uint32_t)bw, rd, bx, by);
+    decode_block<INTERP, KIND>(blocks, w, h, bw, bh, p0, p1, rgba, gid, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,16 +240,16 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t *__restrict__ bloc
 // BC4/BC5 encode and every decoder of a batch: one lane per block, one unit of
 // 256 blocks per workgroup.
 template <int KIND>
-__device__ __forceinline__ void decode_interp(const BatchJob &J, int gid) {
+__device__ __forceinline__ void decode_interp(const BatchJob &J, int gid, int bx, int by) {
     switch (J.interp) {
         case QTX_INTERP_IDEAL_ROUND:
-            decode_block<QTX_INTERP_IDEAL_ROUND, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid);
+            decode_block<QTX_INTERP_IDEAL_ROUND, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid, bx, by);
             break;
         case QTX_INTERP_NVIDIA:
-            decode_block<QTX_INTERP_NVIDIA, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid);
+            decode_block<QTX_INTERP_NVIDIA, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid, bx, by);
             break;
-        case QTX_INTERP_AMD: decode_block<QTX_INTERP_AMD, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid); break;
-        default: decode_block<QTX_INTERP_IDEAL, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid); break;
+        case QTX_INTERP_AMD: decode_block<QTX_INTERP_AMD, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid, bx, by); break;
+        default: decode_block<QTX_INTERP_IDEAL, KIND>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, gid, bx, by); break;
     }
 }
 
@@ -261,13 +260,14 @@ __global__ __launch_bounds__(256) void k_light_batch(const BatchJob *__restrict_
         const BatchJob &J = unit_job(jobs, njobs, u);
         const uint32_t gid = (u - J.unit_begin) * 256u + threadIdx.x;
         if (gid >= J.nblocks) continue;
+        const int bx = (int)(gid % (uint32_t)J.bw), by = (int)(gid / (uint32_t)J.bw);
         switch (J.kind) {
-            case BJ_ENC_BC4: encode_bc45_block<false>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
-            case BJ_ENC_BC5: encode_bc45_block<true>(J.src, J.w, J.h, J.bw, (int)gid, J.p0, J.p1, J.dst); break;
-            case BJ_DEC_BC1: decode_interp<DEC_BC1>(J, (int)gid); break;
-            case BJ_DEC_BC3: decode_interp<DEC_BC3>(J, (int)gid); break;
-            case BJ_DEC_BC4: decode_block<0, DEC_BC4>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
-            case BJ_DEC_BC5: decode_block<0, DEC_BC5>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid); break;
+            case BJ_ENC_BC4: encode_bc45_block<false>(J.src, J.w, J.h, (int)gid, bx, by, J.p0, J.p1, J.dst); break;
+            case BJ_ENC_BC5: encode_bc45_block<true>(J.src, J.w, J.h, (int)gid, bx, by, J.p0, J.p1, J.dst); break;
+            case BJ_DEC_BC1: decode_interp<DEC_BC1>(J, (int)gid, bx, by); break;
+            case BJ_DEC_BC3: decode_interp<DEC_BC3>(J, (int)gid, bx, by); break;
+            case BJ_DEC_BC4: decode_block<0, DEC_BC4>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid, bx, by); break;
+            case BJ_DEC_BC5: decode_block<0, DEC_BC5>(J.src, J.w, J.h, J.bw, J.bh, J.p0, J.p1, J.dst, (int)gid, bx, by); break;
             default: break;
         }
     }
@@ -423,10 +423,11 @@ hipError_t launch_bc4_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
     const dim3 grid(grid_for(nblocks, kThreads)), block(kThreads);
+    const dev::RowDiv rd = dev::make_row_div((uint32_t)bw);
     if (bc5)
-        hipLaunchKernelGGL((dev::k_bc4_encode<true>), grid, block, 0, stream, d_rgba, w, h, bw, nblocks, ch0, ch1, d_out);
+        hipLaunchKernelGGL((dev::k_bc4_encode<true>), grid, block, 0, stream, d_rgba, w, h, bw, rd, nblocks, ch0, ch1, d_out);
     else
-        hipLaunchKernelGGL((dev::k_bc4_encode<false>), grid, block, 0, stream, d_rgba, w, h, bw, nblocks, ch0, ch1, d_out);
+        hipLaunchKernelGGL((dev::k_bc4_encode<false>), grid, block, 0, stream, d_rgba, w, h, bw, rd, nblocks, ch0, ch1, d_out);
     return hipGetLastError();
 }
 
@@ -434,8 +435,9 @@ hipError_t launch_decode(int format, const uint8_t *d_blocks, int w, int h, uint
                          hipStream_t stream) {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const dim3 grid(grid_for((long long)bw * bh, kThreads)), block(kThreads);
+    const dev::RowDiv rd = dev::make_row_div((uint32_t)bw);
 #define QTX_DEC(I, K) \
-    hipLaunchKernelGGL((dev::k_decode<I, K>), grid, block, 0, stream, d_blocks, w, h, bw, bh, p0, p1, d_rgba)
+    hipLaunchKernelGGL((dev::k_decode<I, K>), grid, block, 0, stream, d_blocks, w, h, bw, bh, rd, p0, p1, d_rgba)
     switch (format) {
         case QTX_FMT_BC4: QTX_DEC(0, dev::DEC_BC4); break;
         case QTX_FMT_BC5: QTX_DEC(0, dev::DEC_BC5); break;

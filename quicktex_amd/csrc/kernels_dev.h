@@ -27,6 +27,28 @@ __device__ __forceinline__ uint32_t load_texel(const uint8_t *__restrict__ rgba,
     return *reinterpret_cast<const uint32_t *>(rgba + 4 * ((size_t)y * (size_t)w + (size_t)x));
 }
 
+// Division of a block index by the texture's width in blocks without a divide
+// sequence (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, theorem 4.2 with N = 31 dividend bits): for
+// 1 < bw < 2^31, l = ceil(log2 bw), m = floor(2^(31+l) / bw) + 1 (below 2^32) and
+// s = l - 1, floor(gid / bw) = mulhi(gid, m) >> s for every gid < 2^31; m = 0
+// stands for bw = 1.
+struct RowDiv {
+    uint32_t m, s;
+};
+inline RowDiv make_row_div(uint32_t bw) {
+    if (bw <= 1) return RowDiv{0u, 0u};
+    uint32_t l = 0;
+    while ((1ull << l) < bw) l++;
+    return RowDiv{(uint32_t)((1ull << (31 + l)) / bw + 1), l - 1};
+}
+// (bx, by) of block gid of a texture bw blocks wide
+__device__ __forceinline__ void block_xy(uint32_t gid, uint32_t bw, RowDiv d, int &bx, int &by) {
+    const uint32_t y = d.m ? __umulhi(gid, d.m) >> d.s : gid;
+    by = (int)y;
+    bx = (int)(gid - y * bw);
+}
+
 // High 32 bits of the product of two 24-bit values (v_mul_hi_u32_u24, a full-rate
 // instruction; the 32-bit v_mul_hi_u32 is not).
 __device__ __forceinline__ uint32_t mul_hi_u24(uint32_t a, uint32_t b) {
