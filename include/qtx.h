@@ -111,9 +111,10 @@ qtx_status qtx_bc1_encode_device(qtx_ctx *ctx, const qtx_bc1_params *params, con
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The same encode and bytes as qtx_bc1_encode_device, with
- * the queued lane kernel's work-queue words and park records in a static array
- * of the library (8 MiB + 1 KiB per GPU, held from the library's load until the
- * process ends) instead of the context's lent sets; it calls no allocator.
+ * the lane kernels' work-queue words and their park records or block lists in a
+ * static array of the library (8 MiB + 1 KiB per GPU, held from the library's
+ * load until the process ends) instead of the context's lent sets; it calls no
+ * allocator.
  * Textures of at most QTX_NOALLOC_MAX_BLOCKS blocks (ceil(w/4) * ceil(h/4)), e.g.
  * 2048 x 2048; larger ones return QTX_ERANGE.  Every call on one GPU uses that
  * GPU's one array, whatever context or stream it comes from: calls whose work

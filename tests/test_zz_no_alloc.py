@@ -4,9 +4,10 @@ Called through the C ABI and compared, byte for byte, with qtx_bc1_encode_device
 on the same input: two calls in a row on one stream, the second with a smaller
 texture (the static scratch keeps the first call's contents), both at sizes and
 settings that take the queued lane kernel and park endpoint searches (level 18),
-then a setting that takes the static lane kernel and one below the lane
-crossovers (the 16-lane encoder; no scratch is touched).  Sizes past the
-maximum are refused before any device work.
+then a setting that takes the static lane kernel over its block lists (kept in
+the same scratch) and one below the lane crossovers (the 16-lane encoder; no
+scratch is touched).  Sizes past the maximum are refused before any device
+work.
 """
 import numpy as np
 import pytest
@@ -37,7 +38,7 @@ def test_noalloc_equals_device_entry_point(qtx):
     try:
         cases = [((2048, 2048), 18, 0x13, 2),  # the maximum: queued kernel, parked searches
                  ((1798, 1602), 18, 0x13, 0),  # smaller, ragged: the scratch holds the last call's words
-                 ((1024, 512), 9, 4, 1),       # static lane kernel
+                 ((1024, 512), 9, 4, 1),       # static lane kernel over block lists
                  ((256, 64), 18, 0x13, 3)]     # 16-lane encoder
         for (w, h), level, mode, interp in cases:
             print('noalloc case %dx%d level %d mode %#x interp %d' % (w, h, level, mode, interp), flush=True)
