@@ -97,9 +97,9 @@ __device__ __forceinline__ void encode_lane_block(const uint8_t *__restrict__ rg
 // list_capacity(nblocks) words from q x that capacity.  The encode's workgroup e
 // takes entries of list e % kLaneQueues (workgroups go to the XCDs round-robin, so
 // a list is split and encoded on one XCD).  kLaneRetireWord counts the encode
-// workgroups that have read their count; the last of them returns the counts and
-// itself to zero, so the next launch on the set (stream-ordered after this one)
-// starts from zero.
+// workgroups that are done (each read its count first); the last of them returns
+// the counts and itself to zero, so the next launch on the set (stream-ordered
+// after this one) starts from zero.
 template <int INTERP, bool BC3>
 __global__ __launch_bounds__(256) void k_bc1_split(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                    uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
@@ -138,38 +138,40 @@ __global__ __launch_bounds__(256) void k_bc1_split(const uint8_t *__restrict__ r
 
 // BC1/BC3 encode, one block per lane (any settings; LIGHT: the settings of
 // levels 0-12 as constants, lane::light_settings).  LIST: the blocks k_bc1_split
-// listed (the grid covers every block; workgroups past the list's end return at
-// once).
+// listed (the grid covers every list's capacity; workgroups past a list's end
+// encode nothing).
 template <int INTERP, bool BC3, bool LIGHT, bool LIST>
 __global__ __launch_bounds__(256, 4) void k_bc1_lane(const uint8_t *__restrict__ rgba, int w, int h, int bw, int nblocks,
                                                   uint8_t *__restrict__ out, const DeviceTables *__restrict__ T,
                                                   BC1Settings s_in, uint32_t *__restrict__ queue,
                                                   const uint32_t *__restrict__ list) {
     const BC1Settings s = lane::fixed<LIGHT ? lane::kFixLight : lane::kFixNone>(s_in);
-    uint32_t count = (uint32_t)nblocks, i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t *entries = nullptr;
     if constexpr (LIST) {
+        // the count, this lane's entry and the tables are read together; the entry
+        // (inside the list's capacity, so always in bounds) is used only below the
+        // count
         __shared__ uint32_t list_count;
         const uint32_t q = blockIdx.x % kLaneQueues;
-        if (threadIdx.x == 0) {
-            // the acquire orders the read before the retire count: the last
-            // workgroup's reset cannot overtake it
-            list_count = __hip_atomic_load(&queue[q * kLaneQueueStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (atomicAdd(&queue[kLaneRetireWord], 1u) == gridDim.x - 1) {
-                for (int k = 0; k < kLaneQueues; k++) atomicExch(&queue[k * kLaneQueueStride], 0u);
-                atomicExch(&queue[kLaneRetireWord], 0u);
-            }
-        }
+        const uint32_t first = (blockIdx.x / kLaneQueues) * blockDim.x, i = first + threadIdx.x;
+        if (threadIdx.x == 0)
+            list_count = __hip_atomic_load(&queue[q * kLaneQueueStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t gid = list[(size_t)q * list_capacity((uint32_t)nblocks) + i];
+        lane::load_tables<INTERP, false>(T);
         __syncthreads();
-        count = list_count;
-        const uint32_t first = (blockIdx.x / kLaneQueues) * blockDim.x;
-        if (first >= count) return;  // the whole workgroup
-        i = first + threadIdx.x;
-        entries = list + (size_t)q * list_capacity((uint32_t)nblocks);
+        const uint32_t count = list_count;
+        if (first < count && i < count) encode_lane_block<INTERP>(rgba, w, h, bw, (int)gid, out, T, s, BC3);
+        // every workgroup counts itself once it has its count (thread 0 read it
+        // above, before the barrier); the last returns the counts to zero
+        if (threadIdx.x == 0 && atomicAdd(&queue[kLaneRetireWord], 1u) == gridDim.x - 1) {
+            for (int k = 0; k < kLaneQueues; k++) atomicExch(&queue[k * kLaneQueueStride], 0u);
+            atomicExch(&queue[kLaneRetireWord], 0u);
+        }
+    } else {
+        lane::load_tables<INTERP, false>(T);
+        __syncthreads();
+        const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        if (gid < nblocks) encode_lane_block<INTERP>(rgba, w, h, bw, gid, out, T, s, BC3);
     }
-    lane::load_tables<INTERP, false>(T);
-    __syncthreads();
-    if (i < count) encode_lane_block<INTERP>(rgba, w, h, bw, LIST ? (int)entries[i] : (int)i, out, T, s, BC3);
 }
 
 // queue[0] hands out units and queue[1] counts the waves that found it empty;

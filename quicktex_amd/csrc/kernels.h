@@ -67,11 +67,11 @@ constexpr int kLaneQueueWords = 256;  // per queue, rounded up (1 KiB)
 // 12, 18 k, 0.525 against 0.585; level 13, 34 k, 0.542 against 0.533).
 constexpr double kLaneQueueMinOps = 20000.0;
 // The static kernel lists the non-solid blocks first from this estimated cost per
-// block up; on cheaper settings the listing pass costs more than the idle lanes
-// it saves (same texture: level 7, 4 k, 0.180 ms listed against 0.177 plain;
-// level 8, 6 k, 0.233 against 0.242; level 10, 12 k, 0.376 against 0.420; level
-// 0, 2 k, 0.120 against 0.105).
-constexpr double kLaneListMinOps = 5000.0;
+// block up; on cheaper settings the listing pass costs about what the idle lanes
+// it saves do (same texture: level 7, 4 k, 0.166 ms listed against 0.177 plain;
+// level 8, 6 k, 0.219 against 0.240; level 10, 12 k, 0.363 against 0.414; levels
+// 2-6, 2.5 k, even; level 0, 2 k, 0.109 against 0.105).
+constexpr double kLaneListMinOps = 3000.0;
 // The encoder a launch of this texture takes (Bc1Route of bc1_launch.h; the
 // queued route falls back to the static lane kernel when no queue set is lent).
 int bc1_route(const BC1Settings &s, int w, int h, int path);

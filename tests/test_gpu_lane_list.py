@@ -1,7 +1,7 @@
 """GPU: the static BC1/BC3 lane encoder over its block list.
 
 Settings below the queued kernel's cost (levels 0-12 and other settings without
-long searches) run on the static lane kernel; from level 8's cost up, and with a
+long searches) run on the static lane kernel; from level 7's cost up, and with a
 queue set lent to the launch, a first kernel stores the solid blocks and lists
 the others, and the lane kernel encodes the listed blocks one per lane.  The
 lists' counts and the encode's read count live in the lent queue words and must
