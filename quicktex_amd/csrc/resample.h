@@ -30,7 +30,8 @@ namespace qtx {
 constexpr int kMaxMipLevels = 32;  // a 2^31-texel texture has at most 32 levels
 constexpr int kResampleWideTaps = 64;  // windows longer than this use a wave per output
 constexpr int kRowMaxWidth = 8192;     // widest base whose rows the horizontal pass keeps in LDS
-constexpr int kRowGrid = 2048;         // workgroups of that pass (they stride over the rows)
+constexpr int kRowGrid = 1024;         // workgroups of that pass (they stride over the rows)
+constexpr int kBoxMaxWidth = 4096;     // widest base whose rows also get box sums in LDS (k_mip_hrows)
 
 // Layout of one mip chain in a caller's workspace, and its launch ranges.  Passed
 // to the kernels by value.
@@ -47,6 +48,7 @@ struct MipPlan {
     uint32_t h0[kMaxMipLevels + 1], v0[kMaxMipLevels + 1];
     int32_t gx[kMaxMipLevels];        // lanes per output in the horizontal pass over LDS rows
     uint32_t exact_x, exact_y;        // bit m: that axis of level m shrinks by a power of two
+    uint32_t boxes_x;                 // bit m: level m's interior outputs come from box sums (k_mip_hrows)
     uint8_t *dst[kMaxMipLevels];  // level outputs, RGBA8 (dst[0] unused)
 };
 

@@ -237,17 +237,106 @@ __global__ __launch_bounds__(256) void k_mip_pass(MipPlan p, const uint8_t *__re
 // l, l + G, ...; the group adds its partial sums with DPP lane moves (integer
 // sums: any order gives the same total).
 // Rows of at most kRowMaxWidth texels.
+//
+// Box sums.  A shrink by r = 2^a (16 <= r <= 1024) gives an interior output o the
+// window [(2o - 1) h, (2o + 3) h), h = r / 2, with the weights (2t + 1) << sh
+// rising over its first 2h taps and (4r - 2t - 1) << sh falling over the last
+// 2h (sh = 21 - 2a; see k_mip_hrows).  Over the four boxes of h texels it spans,
+// q = 2o - 1 .. 2o + 2, with S = the box's texel sum and M = the sum of its texels
+// times their offset in the box, that weighted sum is exactly
+//   2^sh (2 M_q + S_q + 2 M_q+1 + (2h + 1) S_q+1 + (4h - 1) S_q+2 - 2 M_q+2
+//         + (2h - 1) S_q+3 - 2 M_q+3),
+// so such an output costs a handful of operations whatever its window, once the
+// row's box sums are known; those of width 2h come from those of width h (S as
+// the sum of the two halves, M as M_lo + M_hi + h S_hi), level after level.  All
+// terms are non-negative integers below 2^31: the weighted sum is at most
+// 255 x 2^22.  The sums live in LDS after the row, per box S of the four channels
+// then M of the four (boxes_x marks the levels that use them; rows of at most
+// kBoxMaxWidth texels).
 extern __shared__ __align__(16) uint32_t mip_lds[];
+
+// Box sums of width 8 straight from the row: box b of nb at out[c * nb + b] (S) and
+// out[(4 + c) * nb + b] (M), c the channel.
+__device__ __forceinline__ void boxes_from_row(const uint32_t *__restrict__ row, int nb, int32_t *__restrict__ out) {
+    for (int b = (int)threadIdx.x; b < nb; b += 256) {
+        int32_t ss[4] = {0, 0, 0, 0}, mm[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t px = row[8 * b + i];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int32_t v = (int32_t)((px >> (8 * c)) & 0xFFu);
+                ss[c] += v;
+                mm[c] += i * v;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            out[c * nb + b] = ss[c];
+            out[(4 + c) * nb + b] = mm[c];
+        }
+    }
+}
+
+// Box sums of width 2h (nb boxes) from those of width h (2 nb boxes).
+__device__ __forceinline__ void boxes_double(const int32_t *__restrict__ in, int h, int nb, int32_t *__restrict__ out) {
+    for (int b = (int)threadIdx.x; b < nb; b += 256) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int32_t slo = in[c * 2 * nb + 2 * b], shi = in[c * 2 * nb + 2 * b + 1];
+            const int32_t mlo = in[(4 + c) * 2 * nb + 2 * b], mhi = in[(4 + c) * 2 * nb + 2 * b + 1];
+            out[c * nb + b] = slo + shi;
+            out[(4 + c) * nb + b] = mlo + mhi + h * shi;
+        }
+    }
+}
+
+// Interior output o of a shrink by r = 2h = 2^a from the box sums of width h.
+__device__ __forceinline__ uint32_t box_output(const int32_t *__restrict__ bx, int nb, int h, int sh, int o) {
+    const int q = 2 * o - 1;
+    int32_t ss[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int32_t *S = bx + c * nb + q, *M = bx + (4 + c) * nb + q;
+        const int32_t t = 2 * M[0] + S[0] + 2 * M[1] + (2 * h + 1) * S[1] + (4 * h - 1) * S[2] - 2 * M[2] +
+                          (2 * h - 1) * S[3] - 2 * M[3];
+        ss[c] = (t << sh) + (1 << (kBits - 1));
+    }
+    return pack_clip(ss);
+}
 
 __global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__restrict__ base,
                                                    uint8_t *__restrict__ ws) {
     uint32_t *row = mip_lds;
+    // box sums, two buffers used in turn: width 8 (p.w words), then 16 (p.w / 2)
+    int32_t *const boxbuf[2] = {reinterpret_cast<int32_t *>(mip_lds + p.w),
+                                reinterpret_cast<int32_t *>(mip_lds + 2 * p.w)};
     const int32_t *wsi = reinterpret_cast<const int32_t *>(ws);
     const int tid = (int)threadIdx.x;
+    // the workgroup's next row is read into registers while it works on the
+    // current one (kRowPrefetch texels per thread; wider rows read the rest then)
+    constexpr int kRowPrefetch = kBoxMaxWidth / 256;
+    uint32_t ahead[kRowPrefetch];
+    auto fetch = [&](int y) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(base) + (size_t)y * p.w;
+#pragma unroll
+        for (int k = 0; k < kRowPrefetch; k++) {
+            const int x = tid + 256 * k;
+            ahead[k] = x < p.w ? src[x] : 0u;
+        }
+    };
+    if ((int)blockIdx.x < p.h) fetch((int)blockIdx.x);
     for (int y = (int)blockIdx.x; y < p.h; y += (int)gridDim.x) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(base) + (size_t)y * p.w;
-        for (int x = tid; x < p.w; x += 256) row[x] = src[x];
+#pragma unroll
+        for (int k = 0; k < kRowPrefetch; k++) {
+            const int x = tid + 256 * k;
+            if (x < p.w) row[x] = ahead[k];
+        }
+        for (int x = tid + 256 * kRowPrefetch; x < p.w; x += 256) row[x] = src[x];
         __syncthreads();
+        if (y + (int)gridDim.x < p.h) fetch(y + (int)gridDim.x);
+        int box_h = 0, box_cur = 1;  // width of the box sums built so far (0: none) and their buffer
         for (int m = 1; m < p.levels; m++) {
             const int lw = p.lw[m];
             if (lw == p.w) continue;
@@ -262,11 +351,26 @@ __global__ __launch_bounds__(256) void k_mip_hrows(MipPlan p, const uint8_t *__r
             const bool closed = ((p.exact_x >> m) & 1u) && r <= 1024;
             const int sh = closed ? 21 - 2 * __builtin_ctz((unsigned)r) : 0;
             uint32_t *dst = reinterpret_cast<uint32_t *>(p.lh[m] != p.h ? ws + p.tmp[m] : p.dst[m]) + (size_t)y * lw;
-            const int lanes = lw * G;
+            // interior outputs 1 .. lw - 2 from box sums of width r / 2 (uniform)
+            const bool boxes = ((p.boxes_x >> m) & 1u) && (r / 2 == 8 || r / 2 == 2 * box_h);
+            if (boxes) {
+                const int h = r / 2, nb = p.w / h;
+                int32_t *next = boxbuf[box_cur ^ 1];
+                if (h == 8) boxes_from_row(row, nb, next);
+                else boxes_double(boxbuf[box_cur], h / 2, nb, next);
+                __syncthreads();
+                box_cur ^= 1;
+                box_h = h;
+                for (int o = 1 + tid; o < lw - 1; o += 256) dst[o] = box_output(next, nb, h, sh, o);
+            }
+            // the rest: every output, or with box sums only the two edge outputs
+            const int nout = boxes ? 2 : lw;
+            const int lanes = nout * G;
             for (int b = 0; b < lanes; b += 256) {  // uniform across the workgroup
                 const int it = b + tid;
-                const int o = it >> gshift, l = it & (G - 1);
-                const bool act = o < lw;  // uniform within a group
+                const int k = it >> gshift, l = it & (G - 1);
+                const int o = boxes ? (k == 0 ? 0 : lw - 1) : k;
+                const bool act = k < nout;  // uniform within a group
                 int32_t ss[4] = {0, 0, 0, 0};
                 if (act) {
                     const int lo = win[2 * o], n = win[2 * o + 1];
@@ -388,6 +492,11 @@ size_t mip_plan(int32_t w, int32_t h, int32_t levels, MipPlan &p) {
         const double scale_x = (double)w / (double)p.lw[m];
         p.gx[m] = scale_x >= 256.0 ? 64 : scale_x >= 16.0 ? 16 : 1;
         if (power_of_two_scale(h, p.lh[m])) p.exact_y |= 1u << m;
+        // box sums for the interior outputs of power-of-two shrinks by 16 .. 1024
+        // (below 16 the taps cost no more than the sums), on rows that fit
+        const int r = w / p.lw[m];
+        if (m > 0 && ((p.exact_x >> m) & 1u) && r >= 16 && r <= 1024 && p.lw[m] >= 3 && w <= kBoxMaxWidth)
+            p.boxes_x |= 1u << m;
     }
     uint32_t coef = 0, hu = 0, vu = 0;
     for (int m = 0; m < levels; m++) {
@@ -441,7 +550,8 @@ hipError_t launch_mips(const MipPlan &p, const uint8_t *d_base, uint8_t *d_ws, h
                        reinterpret_cast<int32_t *>(d_ws));
     if (p.h0[p.levels]) {
         if (p.w <= kRowMaxWidth) {
-            const size_t lds = (size_t)p.w * 4;
+            // the row, and with box sums their two buffers (p.w and p.w / 2 words)
+            const size_t lds = (size_t)p.w * 4 + (p.boxes_x ? (size_t)p.w * 6 : 0);
             hipLaunchKernelGGL(dev::k_mip_hrows, dim3((unsigned)std::min(p.h, kRowGrid)), dim3(256), lds, stream, p,
                                d_base, d_ws);
         } else {
