@@ -43,6 +43,10 @@ level 10: the resample kernels alone (HBM GB/s), resample + batched encode
 device-resident, and host to host; the reference's flow (Pillow resize + its
 BC3 encoder per level) timed on a 1024x1024 crop and checked bit-exact.
 
+sweep: every BC1 level (FourColor, Ideal) on the 4096x4096 mixture, one plain
+launch each, with its fraction of the VALU roofline, and the HBM-bound kernels
+(BC4/BC5 encode, the four decoders) at 4096x4096 with their fraction of HBM.
+
 Prints one JSON line.
 """
 import argparse
@@ -930,7 +934,7 @@ def main():
                     help='untimed steps first (the shader clock ramps up from idle over the first ~50 ms)')
     ap.add_argument('--cpu-rows', type=int, default=2048, help='texture rows timed on the CPU (multiple of 4)')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--legs', default='plain,c0,c1,batched,served,dds',
+    ap.add_argument('--legs', default='plain,c0,c1,batched,served,dds,sweep',
                     help='comma list of: plain, c0, c1, batched, served, dds, sweep')
     ap.add_argument('--serve-requests', type=int, default=2000)
     ap.add_argument('--serve-rate', type=float, default=0.0,
