@@ -506,7 +506,7 @@ constexpr size_t kBc1Lds = sizeof(dev::Lds);
 constexpr size_t kLaneLds = 0;
 
 template <int I, bool B> void launch_queue(const Bc1Launch &a) {
-    const dim3 grid(a.grid), block(dev::lane::kQueueThreads);
+    const dim3 grid(a.grid), block(a.threads);
     switch (a.fix) {
         case dev::lane::kFixHQ:
             hipLaunchKernelGGL((dev::k_bc1_lane_queue<I, B, dev::lane::kFixHQ>), grid, block, 0, a.stream, a.rgba, a.w,
@@ -522,8 +522,8 @@ template <int I, bool B> void launch_queue(const Bc1Launch &a) {
             break;
     }
     if (a.drain)
-        hipLaunchKernelGGL((dev::k_bc1_search_drain<I, B>), dim3(a.drain_grid), block, 0, a.stream, a.rgba, a.w, a.h,
-                           a.bw, a.out, a.tables, a.s, a.park, a.queue);
+        hipLaunchKernelGGL((dev::k_bc1_search_drain<I, B>), dim3(a.drain_grid), dim3(dev::lane::kQueueThreads), 0,
+                           a.stream, a.rgba, a.w, a.h, a.bw, a.out, a.tables, a.s, a.park, a.queue);
 }
 
 template <int I, bool B, bool LIGHT> void launch_lane_build(const Bc1Launch &a) {

@@ -30,6 +30,7 @@ struct Bc1Launch {
     bool bc3;
     int route;      // Bc1Route
     unsigned grid;  // workgroups
+    int threads;    // kRouteQueue: threads per workgroup (768, or 256 for small launches)
     bool light;     // kRouteLane: the level 0-12 build
     int fix;        // kRouteQueue: the encoder build (dev::lane::Fixed)
     bool drain;     // kRouteQueue: the drain kernel follows (searches park at `stop`)

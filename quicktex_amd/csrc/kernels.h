@@ -39,6 +39,15 @@ inline bool lane_list_env() {
     }();
     return on;
 }
+// Launches of the queued kernel with at most a block per lane of 4-wave
+// workgroups on every CU take such workgroups, unless QTX_QUEUE_SMALL=0.
+inline bool queue_small_env() {
+    static const bool on = [] {
+        const char *e = std::getenv("QTX_QUEUE_SMALL");
+        return !(e && !std::strcmp(e, "0"));
+    }();
+    return on;
+}
 // Estimated ALU ops per BC1 block for the encoder settings.
 double bc1_block_ops(const BC1Settings &s);
 // Whether the lane-per-block encoder serves settings s when `blocks` blocks of

@@ -377,9 +377,14 @@ hipError_t launch_bc1_encode(const uint8_t *d_rgba, int w, int h, uint8_t *d_out
     const int route = bc1_route(s, w, h, path);
     if (route != kRouteGroup) {
         if (d_queue && route == kRouteQueue) {
-            // persistent: one workgroup of 12 waves per CU, fewer for small launches
+            // persistent: one workgroup per CU (its LDS takes the CU's), of 12 waves, or
+            // of 4 when the launch has at most a block per lane that way: a block then
+            // runs on a wave alone on its SIMD (a third of the time it takes among 3)
+            // and the blocks spread over every CU
             a.route = kRouteQueue;
-            a.grid = (unsigned)std::min<long long>(grid_for(nblocks, dev::lane::kQueueThreads), std::max(cu_count, 1));
+            const int cus = std::max(cu_count, 1);
+            a.threads = nblocks <= (long long)cus * kThreads && queue_small_env() ? kThreads : dev::lane::kQueueThreads;
+            a.grid = (unsigned)std::min<long long>(grid_for(nblocks, a.threads), cus);
             // parking (and the drain launch after the queued kernel) only with a park buffer
             a.drain = d_park && bc1_park_bytes(s, w, h, path) > 0;
             a.stop = a.drain ? park_stop_env() : 0xFFFFFFFFu;
