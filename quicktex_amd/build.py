@@ -31,7 +31,7 @@ HIP_FLAGS = [
     '-Wall', '-Wno-unused-function', '-Wno-unknown-pragmas',
     '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC,
 ]
-HIP_SOURCES = ['kernels.hip', 'bc1_ideal.hip', 'bc1_ideal_round.hip', 'bc1_nvidia.hip', 'bc1_amd.hip', 'resample.hip', 'capi.cpp', 'qtx_tables.cpp', 'batch.cpp', 'server.cpp', 'mips.cpp']
+HIP_SOURCES = ['kernels.hip', 'bc1_ideal.hip', 'bc1_ideal_round.hip', 'bc1_nvidia.hip', 'bc1_amd.hip', 'resample.hip', 'capi.cpp', 'qtx_tables.cpp', 'batch.cpp', 'server.cpp', 'mips.cpp', 'staging.cpp']
 
 
 def _run(cmd, verbose):

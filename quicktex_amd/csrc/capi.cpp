@@ -446,7 +446,8 @@ qtx_status qtx_bc5_decode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, c
 }
 
 // ---------------------------------------------------------------------------
-// host entry points: stage through the context's device buffers
+// host entry points: stage through the context's device buffers (large transfers
+// through its pinned chunks, staging.h)
 
 }  // extern "C"
 
@@ -463,14 +464,13 @@ qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *
     qtx_status st = ensure(ctx->d_in, ctx->in_cap, in_bytes);
     if (st == QTX_OK) st = ensure(ctx->d_out, ctx->out_cap, out_need);
     if (st != QTX_OK) return st;
-    QTX_HIP(hipMemcpyAsync(ctx->d_in, in, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    QTX_HIP(ctx->stager.upload(ctx->d_in, in, in_bytes, ctx->stream));
     st = launch(ctx->d_in, ctx->d_out, (void *)ctx->stream);
     if (st != QTX_OK) {
         (void)hipStreamSynchronize(ctx->stream);
         return st;
     }
-    QTX_HIP(hipMemcpyAsync(out, ctx->d_out, out_need, hipMemcpyDeviceToHost, ctx->stream));
-    QTX_HIP(hipStreamSynchronize(ctx->stream));
+    QTX_HIP(ctx->stager.download(out, ctx->d_out, out_need, ctx->stream));
     return QTX_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "qtx_internal.h"
+#include "staging.h"
 
 // A device, its work stream, its copy of the constant tables and the staging
 // buffers of the host-memory entry points.
@@ -23,6 +24,7 @@ struct qtx_ctx {
     uint8_t *d_out = nullptr;
     size_t out_cap = 0;
     uint32_t *d_flag = nullptr;  // result word of the alpha test
+    qtx::rt::Stager stager;      // pinned staging of large host transfers (under mu)
     std::mutex mu;
     // Work-queue words of the queued BC1/BC3 lane kernel: kQueueSlots sets, each
     // lent to one launch at a time.  A set is free again once the event recorded

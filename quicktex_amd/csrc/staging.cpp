@@ -1,0 +1,140 @@
+// staging.cpp -- the pinned, chunked host staging of staging.h.
+#include "staging.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace qtx {
+namespace rt {
+
+CopyPool::CopyPool(int workers) {
+    for (int i = 0; i < workers; i++) threads_.emplace_back([this, i] { run(i + 1); });
+}
+
+CopyPool::~CopyPool() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    go_.notify_all();
+    for (std::thread &t : threads_) t.join();
+}
+
+namespace {
+// Part `part` of `parts` of [0, n), page-aligned boundaries.
+void part_range(size_t n, int parts, int part, size_t &b, size_t &e) {
+    size_t per = (n + (size_t)parts - 1) / (size_t)parts;
+    per = (per + 4095) & ~(size_t)4095;
+    b = std::min(n, per * (size_t)part);
+    e = std::min(n, b + per);
+}
+}  // namespace
+
+void CopyPool::run(int part) {
+    uint64_t seen = 0;
+    for (;;) {
+        uint8_t *dst;
+        const uint8_t *src;
+        size_t n;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            dst = dst_;
+            src = src_;
+            n = n_;
+        }
+        size_t b, e;
+        part_range(n, (int)threads_.size() + 1, part, b, e);
+        if (e > b) std::memcpy(dst + b, src + b, e - b);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+}
+
+void CopyPool::copy(uint8_t *dst, const uint8_t *src, size_t n) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        dst_ = dst;
+        src_ = src;
+        n_ = n;
+        pending_ = (int)threads_.size();
+        gen_++;
+    }
+    go_.notify_all();
+    size_t b, e;
+    part_range(n, (int)threads_.size() + 1, 0, b, e);
+    if (e > b) std::memcpy(dst + b, src + b, e - b);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+}
+
+Stager::~Stager() {
+    for (hipEvent_t &ev : ev_)
+        if (ev) {
+            (void)hipEventSynchronize(ev);
+            (void)hipEventDestroy(ev);
+        }
+    if (pin_) (void)hipHostFree(pin_);
+    delete pool_;
+}
+
+hipError_t Stager::init() {
+    if (pin_) return hipSuccess;
+    hipError_t e = hipHostMalloc((void **)&pin_, 2 * kChunk, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        pin_ = nullptr;
+        return e;
+    }
+    for (hipEvent_t &ev : ev_)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    pool_ = new CopyPool(kWorkers);
+    return hipSuccess;
+}
+
+hipError_t Stager::upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s) {
+    if (n < kMinBytes || init() != hipSuccess) return hipMemcpyAsync(d_dst, src, n, hipMemcpyHostToDevice, s);
+    size_t k = 0;
+    for (size_t off = 0; off < n; off += kChunk, k++) {
+        const size_t len = std::min(kChunk, n - off);
+        uint8_t *slot = pin_ + (k & 1) * kChunk;
+        // the slot's last DMA (this call's chunk k - 2, or a previous call's) is done
+        hipError_t e = hipEventSynchronize(ev_[k & 1]);
+        if (e != hipSuccess) return e;
+        pool_->copy(slot, src + off, len);
+        if ((e = hipMemcpyAsync(d_dst + off, slot, len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_[k & 1], s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s) {
+    hipError_t e;
+    if (n < kMinBytes || init() != hipSuccess) {
+        if ((e = hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    }
+    const size_t chunks = (n + kChunk - 1) / kChunk;
+    // chunk k goes to slot k & 1; the DMA of chunk k + 1 runs while chunk k is copied out
+    auto enqueue = [&](size_t k) -> hipError_t {
+        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        hipError_t r = hipEventSynchronize(ev_[k & 1]);  // an upload's DMA from that slot may still run
+        if (r == hipSuccess) r = hipMemcpyAsync(pin_ + (k & 1) * kChunk, d_src + off, len, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipEventRecord(ev_[k & 1], s);
+        return r;
+    };
+    if ((e = enqueue(0)) != hipSuccess) return e;
+    for (size_t k = 0; k < chunks; k++) {
+        if ((e = hipEventSynchronize(ev_[k & 1])) != hipSuccess) return e;
+        if (k + 1 < chunks && (e = enqueue(k + 1)) != hipSuccess) return e;
+        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        pool_->copy(dst + off, pin_ + (k & 1) * kChunk, len);
+    }
+    return hipSuccess;
+}
+
+}  // namespace rt
+}  // namespace qtx
