@@ -431,8 +431,8 @@ qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *d
     if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
     std::vector<qtx_job> dev(jobs, jobs + njobs);
     for (size_t i = 0; i < njobs; i++) {
-        QTX_HIP(hipMemcpyAsync(ctx->d_in + src_off[i], jobs[i].src, prep[i].src_bytes, hipMemcpyHostToDevice,
-                               ctx->stream));
+        QTX_HIP(ctx->stager.upload(ctx->d_in + src_off[i], static_cast<const uint8_t *>(jobs[i].src),
+                                   prep[i].src_bytes, ctx->stream));
         dev[i].src = ctx->d_in + src_off[i];
         dev[i].dst = ctx->d_in + dst_off[i];
     }
@@ -443,9 +443,9 @@ qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *d
     }
     st = qtx_batch_launch(b, ctx->stream);
     for (size_t i = 0; i < njobs && st == QTX_OK; i++) {
-        const hipError_t e = hipMemcpyAsync(jobs[i].dst, ctx->d_in + dst_off[i], prep[i].dst_bytes,
-                                            hipMemcpyDeviceToHost, ctx->stream);
-        if (e != hipSuccess) st = hip_fail(e, "hipMemcpyAsync");
+        const hipError_t e = ctx->stager.download(static_cast<uint8_t *>(jobs[i].dst), ctx->d_in + dst_off[i],
+                                                  prep[i].dst_bytes, ctx->stream);
+        if (e != hipSuccess) st = hip_fail(e, "download");
     }
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");

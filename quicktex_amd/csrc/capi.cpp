@@ -471,6 +471,7 @@ qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *
         return st;
     }
     QTX_HIP(ctx->stager.download(out, ctx->d_out, out_need, ctx->stream));
+    QTX_HIP(hipStreamSynchronize(ctx->stream));
     return QTX_OK;
 }
 

@@ -83,12 +83,11 @@ qtx_status qtx_mips_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, int32
     total += ws;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
-    QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+    QTX_HIP(ctx->stager.upload(ctx->d_in, rgba, raw_bytes(width, height), ctx->stream));
     for (int m = 1; m < levels; m++) p.dst[m] = ctx->d_in + off[m];
     QTX_HIP(qtx::launch_mips(p, ctx->d_in, ctx->d_in + ws_off, ctx->stream));
     for (int m = 1; m < levels; m++)
-        QTX_HIP(hipMemcpyAsync(out_levels[m - 1], ctx->d_in + off[m], raw_bytes(p.lw[m], p.lh[m]),
-                               hipMemcpyDeviceToHost, ctx->stream));
+        QTX_HIP(ctx->stager.download(out_levels[m - 1], ctx->d_in + off[m], raw_bytes(p.lw[m], p.lh[m]), ctx->stream));
     QTX_HIP(hipStreamSynchronize(ctx->stream));
     return QTX_OK;
 }
@@ -134,7 +133,7 @@ qtx_status encode_chain(qtx_ctx *ctx, const qtx_job &tmpl, const uint8_t *rgba, 
         if (ctx->in_cap < total) return fail(QTX_EINVAL, "staging holds %zu bytes, %zu needed", ctx->in_cap, total);
     } else {
         if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
-        QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+        QTX_HIP(ctx->stager.upload(ctx->d_in, rgba, raw_bytes(width, height), ctx->stream));
     }
     uint8_t *d = ctx->d_in;
     for (int m = 1; m < levels; m++) p.dst[m] = d + img_off[m];
@@ -150,9 +149,8 @@ qtx_status encode_chain(qtx_ctx *ctx, const qtx_job &tmpl, const uint8_t *rgba, 
     }
     st = qtx_batch_launch(b, ctx->stream);
     for (int m = 0; m < levels && st == QTX_OK; m++) {
-        const hipError_t e = hipMemcpyAsync(out_blocks[m], d + blk_off[m], prep[m].dst_bytes, hipMemcpyDeviceToHost,
-                                            ctx->stream);
-        if (e != hipSuccess) st = hip_fail(e, "hipMemcpyAsync");
+        const hipError_t e = ctx->stager.download(out_blocks[m], d + blk_off[m], prep[m].dst_bytes, ctx->stream);
+        if (e != hipSuccess) st = hip_fail(e, "download");
     }
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
@@ -210,7 +208,7 @@ qtx_status qtx_mips_encode_auto_host(qtx_ctx *ctx, const qtx_bc1_params *bc1, co
     if ((st = chain_plan(width, height, levels, p, ws)) != QTX_OK) return st;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if ((st = ensure(ctx->d_in, ctx->in_cap, chain_stage_bytes(width, height, levels))) != QTX_OK) return st;
-    QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+    QTX_HIP(ctx->stager.upload(ctx->d_in, rgba, raw_bytes(width, height), ctx->stream));
     int32_t has_alpha = 0;
     if ((st = alpha_flag(ctx, ctx->d_in, (uint64_t)width * (uint64_t)height, ctx->stream, &has_alpha)) != QTX_OK)
         return st;
@@ -241,7 +239,7 @@ qtx_status qtx_has_alpha_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t width, 
     if (!rgba || !has_alpha) return fail(QTX_EINVAL, "null pointer");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if ((st = ensure(ctx->d_in, ctx->in_cap, raw_bytes(width, height))) != QTX_OK) return st;
-    QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, raw_bytes(width, height), hipMemcpyHostToDevice, ctx->stream));
+    QTX_HIP(ctx->stager.upload(ctx->d_in, rgba, raw_bytes(width, height), ctx->stream));
     return alpha_flag(ctx, ctx->d_in, (uint64_t)width * (uint64_t)height, ctx->stream, has_alpha);
 }
 
@@ -262,9 +260,9 @@ qtx_status qtx_swizzle_bc3nm_host(qtx_ctx *ctx, const uint8_t *rgba, int32_t wid
     std::lock_guard<std::mutex> lk(ctx->mu);
     const size_t n = raw_bytes(width, height);
     if ((st = ensure(ctx->d_in, ctx->in_cap, n)) != QTX_OK) return st;
-    QTX_HIP(hipMemcpyAsync(ctx->d_in, rgba, n, hipMemcpyHostToDevice, ctx->stream));
+    QTX_HIP(ctx->stager.upload(ctx->d_in, rgba, n, ctx->stream));
     QTX_HIP(qtx::launch_swizzle_bc3nm(ctx->d_in, (uint64_t)width * (uint64_t)height, ctx->stream));
-    QTX_HIP(hipMemcpyAsync(out, ctx->d_in, n, hipMemcpyDeviceToHost, ctx->stream));
+    QTX_HIP(ctx->stager.download(out, ctx->d_in, n, ctx->stream));
     QTX_HIP(hipStreamSynchronize(ctx->stream));
     return QTX_OK;
 }

@@ -89,8 +89,16 @@ hipError_t Stager::init() {
         pin_ = nullptr;
         return e;
     }
-    for (hipEvent_t &ev : ev_)
-        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    for (hipEvent_t &ev : ev_) {
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            ev = nullptr;
+            for (hipEvent_t &made : ev_)
+                if (made) (void)hipEventDestroy(made), made = nullptr;
+            (void)hipHostFree(pin_);
+            pin_ = nullptr;
+            return e;
+        }
+    }
     pool_ = new CopyPool(kWorkers);
     return hipSuccess;
 }
@@ -113,10 +121,7 @@ hipError_t Stager::upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStrea
 
 hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s) {
     hipError_t e;
-    if (n < kMinBytes || init() != hipSuccess) {
-        if ((e = hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        return hipStreamSynchronize(s);
-    }
+    if (n < kMinBytes || init() != hipSuccess) return hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s);
     const size_t chunks = (n + kChunk - 1) / kChunk;
     // chunk k goes to slot k & 1; the DMA of chunk k + 1 runs while chunk k is copied out
     auto enqueue = [&](size_t k) -> hipError_t {

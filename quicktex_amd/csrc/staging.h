@@ -52,7 +52,8 @@ class Stager {
     // src is no longer read once it returns.
     hipError_t upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s);
     // Copies n bytes of device memory d_src to host memory dst after the work
-    // already on s; returns when dst holds them.
+    // already on s.  A staged copy (kMinBytes and more) is complete on return;
+    // a smaller one is enqueued on s (dst holds the bytes once s is synchronised).
     hipError_t download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s);
 
   private:
