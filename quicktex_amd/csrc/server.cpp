@@ -55,17 +55,6 @@ constexpr int kQueueWords = 2 * bt::kGroups;  // two words per kernel group
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
 
-bool is_pinned(const void *p) {
-    hipPointerAttribute_t a;
-    std::memset(&a, 0, sizeof a);
-    const hipError_t e = hipPointerGetAttributes(&a, p);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();  // pageable memory reports an error; clear it
-        return false;
-    }
-    return a.type == hipMemoryTypeHost;
-}
-
 struct Req {
     uint64_t ticket = 0;
     qtx_job job{};  // host pointers
@@ -443,8 +432,8 @@ qtx_status qtx_server_submit(qtx_server *s, const qtx_job *job, size_t dst_bytes
     if (dst_bytes < r->prep.dst_bytes)
         return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", dst_bytes, r->prep.dst_bytes);
     (void)hipSetDevice(s->ctx->device);
-    r->dst_pinned = is_pinned(job->dst);
-    r->src_direct = is_pinned(job->src);
+    r->dst_pinned = rt::host_pinned(job->dst);
+    r->src_direct = rt::host_pinned(job->src);
     r->light = r->prep.group == bt::kGroupLight;
     const size_t need = footprint(*r);
     if (need > s->arena.capacity())

@@ -7,6 +7,15 @@
 namespace qtx {
 namespace rt {
 
+bool host_pinned(const void *p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory reports an error; clear it
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 CopyPool::CopyPool(int workers) {
     for (int i = 0; i < workers; i++) threads_.emplace_back([this, i] { run(i + 1); });
 }
@@ -84,7 +93,7 @@ Stager::~Stager() {
 
 hipError_t Stager::init() {
     if (pin_) return hipSuccess;
-    hipError_t e = hipHostMalloc((void **)&pin_, 2 * kChunk, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void **)&pin_, kSlots * kChunk, hipHostMallocDefault);
     if (e != hipSuccess) {
         pin_ = nullptr;
         return e;
@@ -104,39 +113,48 @@ hipError_t Stager::init() {
 }
 
 hipError_t Stager::upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s) {
-    if (n < kMinBytes || init() != hipSuccess) return hipMemcpyAsync(d_dst, src, n, hipMemcpyHostToDevice, s);
-    size_t k = 0;
-    for (size_t off = 0; off < n; off += kChunk, k++) {
+    if (n < kMinBytes || host_pinned(src) || init() != hipSuccess)
+        return hipMemcpyAsync(d_dst, src, n, hipMemcpyHostToDevice, s);
+    for (size_t off = 0; off < n; off += kChunk) {
         const size_t len = std::min(kChunk, n - off);
-        uint8_t *slot = pin_ + (k & 1) * kChunk;
-        // the slot's last DMA (this call's chunk k - 2, or a previous call's) is done
-        hipError_t e = hipEventSynchronize(ev_[k & 1]);
+        const unsigned k = next_++ % kSlots;
+        uint8_t *slot = pin_ + k * kChunk;
+        // the slot's last DMA (kSlots chunks back, in this call or an earlier one) is done
+        hipError_t e = hipEventSynchronize(ev_[k]);
         if (e != hipSuccess) return e;
         pool_->copy(slot, src + off, len);
         if ((e = hipMemcpyAsync(d_dst + off, slot, len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
-        if ((e = hipEventRecord(ev_[k & 1], s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_[k], s)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s) {
+hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s, size_t min_bytes) {
     hipError_t e;
-    if (n < kMinBytes || init() != hipSuccess) return hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s);
+    if (n < min_bytes || host_pinned(dst) || init() != hipSuccess)
+        return hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s);
     const size_t chunks = (n + kChunk - 1) / kChunk;
-    // chunk k goes to slot k & 1; the DMA of chunk k + 1 runs while chunk k is copied out
+    const unsigned first = next_;
+    next_ += (unsigned)chunks;
+    // chunk k goes to slot (first + k) % kSlots; the DMAs of the next kSlots - 1
+    // chunks run while chunk k is copied out
+    auto slot = [&](size_t k) { return (unsigned)(first + k) % kSlots; };
     auto enqueue = [&](size_t k) -> hipError_t {
         const size_t off = k * kChunk, len = std::min(kChunk, n - off);
-        hipError_t r = hipEventSynchronize(ev_[k & 1]);  // an upload's DMA from that slot may still run
-        if (r == hipSuccess) r = hipMemcpyAsync(pin_ + (k & 1) * kChunk, d_src + off, len, hipMemcpyDeviceToHost, s);
-        if (r == hipSuccess) r = hipEventRecord(ev_[k & 1], s);
+        // the slot's last chunk (an upload's DMA, or this call's chunk k - kSlots,
+        // copied out already) is done with it
+        hipError_t r = hipEventSynchronize(ev_[slot(k)]);
+        if (r == hipSuccess) r = hipMemcpyAsync(pin_ + slot(k) * kChunk, d_src + off, len, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipEventRecord(ev_[slot(k)], s);
         return r;
     };
-    if ((e = enqueue(0)) != hipSuccess) return e;
+    for (size_t k = 0; k < chunks && k + 1 < (size_t)kSlots; k++)
+        if ((e = enqueue(k)) != hipSuccess) return e;
     for (size_t k = 0; k < chunks; k++) {
-        if ((e = hipEventSynchronize(ev_[k & 1])) != hipSuccess) return e;
-        if (k + 1 < chunks && (e = enqueue(k + 1)) != hipSuccess) return e;
+        if ((e = hipEventSynchronize(ev_[slot(k)])) != hipSuccess) return e;
+        if (k + kSlots - 1 < chunks && (e = enqueue(k + kSlots - 1)) != hipSuccess) return e;
         const size_t off = k * kChunk, len = std::min(kChunk, n - off);
-        pool_->copy(dst + off, pin_ + (k & 1) * kChunk, len);
+        pool_->copy(dst + off, pin_ + slot(k) * kChunk, len);
     }
     return hipSuccess;
 }

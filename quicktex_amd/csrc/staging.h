@@ -4,9 +4,9 @@
 // Their buffers are the caller's pageable memory.  The runtime moves pageable
 // memory at about half the link's rate (it stages through buffers of its own:
 // 28.5 GB/s measured for 64 MiB host to device, against 57 GB/s from pinned
-// memory), so large transfers go through a pinned double buffer instead, chunk
-// by chunk: a few host threads copy one chunk (72 GB/s with four) while the DMA
-// engine moves the other.
+// memory), so large transfers go through pinned chunks instead: a few host
+// threads copy one chunk (72 GB/s with four) while the DMA engine moves the
+// ones before it.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,6 +19,10 @@
 
 namespace qtx {
 namespace rt {
+
+// True when p lies in page-locked host memory the runtime knows (hipHostMalloc),
+// which the DMA engine reads and writes in place.
+bool host_pinned(const void *p);
 
 // memcpy split over the calling thread and `workers` threads kept for the purpose.
 class CopyPool {
@@ -40,26 +44,37 @@ class CopyPool {
     bool stop_ = false;
 };
 
-// A context's staging: two pinned chunks, their DMA completion events and the
-// copy threads, made on first use.  One caller at a time (the context's host lock).
+// A context's staging: kSlots pinned chunks (64 MiB), their DMA completion events
+// and the copy threads, made on first use.  One caller at a time (the context's
+// host lock).  The threads fill slot after slot while the DMA engine drains them,
+// and a slot is waited for only when it comes round again: an upload of up to 64
+// MiB never waits on the GPU.  (Streams share the few hardware queues, so an event
+// recorded on the copy stream can complete only after kernels of other streams:
+// a wait for one in the middle of an upload could stall it behind an encode.)
+// Host buffers that are pinned already go to the DMA engine directly.
 class Stager {
   public:
-    static constexpr size_t kChunk = (size_t)8 << 20;
-    static constexpr size_t kMinBytes = (size_t)4 << 20;  // smaller transfers take the runtime's path
+    static constexpr int kSlots = 16;
+    static constexpr size_t kChunk = (size_t)4 << 20;
+    static constexpr size_t kMinBytes = (size_t)8 << 20;  // smaller transfers take the runtime's path
     static constexpr int kWorkers = 3;
     ~Stager();
     // Enqueues the copy of n bytes of host memory src to device memory d_dst on s.
     // src is no longer read once it returns.
     hipError_t upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s);
     // Copies n bytes of device memory d_src to host memory dst after the work
-    // already on s.  A staged copy (kMinBytes and more) is complete on return;
-    // a smaller one is enqueued on s (dst holds the bytes once s is synchronised).
-    hipError_t download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s);
+    // already on s.  A staged copy (min_bytes and more, pageable dst) is complete
+    // on return; any other is enqueued on s (dst holds the bytes once s is
+    // synchronised).
+    // min_bytes: the size from which the copy is staged (kMinBytes, or less where
+    // the caller moves one transfer in several parts).
+    hipError_t download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s, size_t min_bytes = kMinBytes);
 
   private:
     hipError_t init();
     uint8_t *pin_ = nullptr;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[kSlots] = {};
+    unsigned next_ = 0;  // the slot the next chunk takes, kept across calls
     CopyPool *pool_ = nullptr;
 };
 
