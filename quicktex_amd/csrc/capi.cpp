@@ -4,9 +4,12 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "kernels.h"
 #include "qtx_internal.h"
@@ -189,6 +192,36 @@ qtx_status ensure(uint8_t *&buf, size_t &cap, size_t need) {
 
 using namespace qtx::rt;
 
+namespace {
+
+// The band resources of a context (qtx_ctx::copy_stream and its events).
+void release_bands(qtx_ctx *ctx) {
+    if (ctx->copy_stream) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
+        ctx->copy_stream = nullptr;
+    }
+    for (hipEvent_t *evs : {ctx->band_up, ctx->band_done})
+        for (int k = 0; k < qtx_ctx::kMaxBands; k++)
+            if (evs[k]) (void)hipEventDestroy(evs[k]), evs[k] = nullptr;
+}
+
+// Makes them, all or nothing; the device is current.
+hipError_t setup_bands(qtx_ctx *ctx) {
+    if (ctx->copy_stream) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < qtx_ctx::kMaxBands && e == hipSuccess; k++) {
+        e = hipEventCreateWithFlags(&ctx->band_up[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->band_done[k], hipEventDisableTiming);
+    }
+    // the stream last: its being set means the events are there
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) release_bands(ctx);
+    return e;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *qtx_last_error(void) { return last_error(); }
@@ -304,6 +337,7 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
         if (p) (void)hipFreeAsync(p, ctx->stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    release_bands(ctx);
     delete ctx;
     return QTX_OK;
 }
@@ -475,6 +509,127 @@ qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *
     return QTX_OK;
 }
 
+// Host encodes of kBandMinBytes of texels and more are cut into bands of block
+// rows.  Band k is uploaded on the copy stream and encoded on ctx->stream once its
+// rows have landed, so the upload of the next bands overlaps this band's encode.
+// Blocks are encoded independently of each other
+// from their own texels (a ragged last block row's padding aside, handled in
+// host_encode), so the bands' blocks are those of one launch over the whole
+// texture.  The blocks are downloaded band by band once every band is enqueued.
+constexpr size_t kBandMinBytes = (size_t)16 << 20;
+// A band's blocks are staged from this size (the runtime's path for pageable
+// memory runs at about half the rate)
+constexpr size_t kBandStageBytes = (size_t)1 << 20;
+// Band sizes in sixteenths of the block rows: the first quarter starts the GPU
+// while the rest is uploaded.  Measured from pageable memory on BC1 4096x4096
+// against one band: level 18 4.64 -> 3.90 ms, level 10 2.47 -> 2.30 ms, level 5
+// 2.07 -> 2.01 ms (2048x2048: level 18 1.56 -> 1.58 ms, level 10 0.74 -> 0.71 ms).
+// More bands leave more launch tails, which one stream does not overlap.
+// QTX_HOST_BANDS ("2,6,8", ...) sets another plan; "16" is one band.
+constexpr int kBandPlan[] = {4, 12};
+
+struct BandPlan {
+    int n = 0;
+    int parts[qtx_ctx::kMaxBands] = {};
+};
+
+BandPlan read_band_plan() {
+    BandPlan p;
+    if (const char *e = std::getenv("QTX_HOST_BANDS")) {
+        int sum = 0;
+        for (const char *c = e; p.n < qtx_ctx::kMaxBands - 1;) {  // one more unit for a ragged last block row
+            char *end;
+            const long v = std::strtol(c, &end, 10);
+            if (end == c || v <= 0 || v > 16) break;
+            p.parts[p.n++] = (int)v;
+            sum += (int)v;
+            if (*end != ',') break;
+            c = end + 1;
+        }
+        if (sum == 16) return p;
+    }
+    p.n = (int)std::size(kBandPlan);
+    std::copy(std::begin(kBandPlan), std::end(kBandPlan), p.parts);
+    return p;
+}
+
+template <class BandLaunch>
+qtx_status host_encode(qtx_ctx *ctx, const uint8_t *rgba, int32_t w, int32_t h, int block_size, uint8_t *out,
+                       size_t out_bytes, BandLaunch &&launch) {
+    const size_t in_bytes = raw_bytes(w, h), out_need = block_bytes(w, h, block_size);
+    const int bh = (h + 3) / 4, full = h / 4;  // block rows; those with four texel rows
+    if (in_bytes < kBandMinBytes || full < 2)
+        return host_call(ctx, rgba, in_bytes, out, out_need, out_bytes,
+                         [&](const uint8_t *di, uint8_t *dout, void *s) { return launch(di, h, dout, s); });
+    if (!ctx) return fail(QTX_EINVAL, "null context");
+    if (!rgba || !out) return fail(QTX_EINVAL, "null buffer");
+    if (out_bytes < out_need) return fail(QTX_EINVAL, "output buffer holds %zu bytes, %zu needed", out_bytes, out_need);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    QTX_HIP(hipSetDevice(ctx->device));
+    const size_t row_in = (size_t)w * 16, row_out = (size_t)((w + 3) / 4) * block_size;
+    qtx_status st = ensure(ctx->d_in, ctx->in_cap, bh * row_in);
+    if (st == QTX_OK) st = ensure(ctx->d_out, ctx->out_cap, out_need);
+    if (st != QTX_OK) return st;
+    QTX_HIP(setup_bands(ctx));
+
+    // Units of work: bands of the block rows with four texel rows, then, for a
+    // height that is not a multiple of 4, the last block row on its own.  The
+    // reference pads that row by wrapping to the texture's top rows (texel row y
+    // reads row y % h, quicktex/Texture.h:107), so its four rows are put together
+    // here, in that order, and encoded as a texture of height 4.
+    struct Unit {
+        const uint8_t *src;
+        size_t in_off, in_len, out_off, out_len;
+        int32_t rows;
+    };
+    Unit units[qtx_ctx::kMaxBands];
+    int n = 0;
+    static const BandPlan plan = read_band_plan();
+    for (int i = 0, sum = 0, r0 = 0; i < plan.n; i++) {
+        sum += plan.parts[i];
+        const int r1 = (int)((long long)full * sum / 16);
+        if (r1 > r0) units[n++] = Unit{rgba + r0 * row_in, r0 * row_in, (r1 - r0) * row_in, r0 * row_out,
+                                       (r1 - r0) * row_out, 4 * (r1 - r0)};
+        r0 = std::max(r0, r1);
+    }
+    std::vector<uint8_t> last;
+    if (full < bh) {
+        last.resize(row_in);
+        const size_t texel_row = (size_t)w * 4;
+        for (int j = 0; j < 4; j++) {
+            const int y = 4 * full + j;
+            std::memcpy(last.data() + j * texel_row, rgba + (size_t)(y < h ? y : y - h) * texel_row, texel_row);
+        }
+        units[n++] = Unit{last.data(), full * row_in, row_in, full * row_out, row_out, 4};
+    }
+
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < n && e == hipSuccess && st == QTX_OK; k++) {
+        const Unit &u = units[k];
+        e = ctx->stager.upload(ctx->d_in + u.in_off, u.src, u.in_len, ctx->copy_stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->band_up[k], ctx->copy_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->band_up[k], 0);
+        if (e != hipSuccess) break;
+        st = launch(ctx->d_in + u.in_off, u.rows, ctx->d_out + u.out_off, (void *)ctx->stream);
+        if (st == QTX_OK) e = hipEventRecord(ctx->band_done[k], ctx->stream);
+    }
+    for (int k = 0; k < n && e == hipSuccess && st == QTX_OK; k++) {
+        const Unit &u = units[k];
+        e = hipStreamWaitEvent(ctx->copy_stream, ctx->band_done[k], 0);
+        if (e == hipSuccess)
+            e = ctx->stager.download(out + u.out_off, ctx->d_out + u.out_off, u.out_len, ctx->copy_stream,
+                                     kBandStageBytes);
+    }
+    // the copy stream's last work waits for every band's encode (or, after an
+    // error, whatever was enqueued ends here); `last` is read until then
+    const hipError_t e_copy = hipStreamSynchronize(ctx->copy_stream);
+    const hipError_t e_run = hipStreamSynchronize(ctx->stream);
+    if (st != QTX_OK) return st;
+    if (e == hipSuccess) e = e_copy != hipSuccess ? e_copy : e_run;
+    if (e != hipSuccess) return hip_fail(e, "banded host encode");
+    return QTX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -483,20 +638,20 @@ qtx_status qtx_bc1_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint
                           int32_t height, uint8_t *out, size_t out_bytes) {
     qtx_status st = check_dims(width, height);
     if (st != QTX_OK) return st;
-    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 8), out_bytes,
-                     [&](const uint8_t *di, uint8_t *dout, void *s) {
-                         return qtx_bc1_encode_device(ctx, params, di, width, height, dout, s);
-                     });
+    return host_encode(ctx, rgba, width, height, 8, out, out_bytes,
+                       [&](const uint8_t *di, int32_t band_h, uint8_t *dout, void *s) {
+                           return qtx_bc1_encode_device(ctx, params, di, width, band_h, dout, s);
+                       });
 }
 
 qtx_status qtx_bc3_encode(qtx_ctx *ctx, const qtx_bc1_params *params, const uint8_t *rgba, int32_t width,
                           int32_t height, uint8_t *out, size_t out_bytes) {
     qtx_status st = check_dims(width, height);
     if (st != QTX_OK) return st;
-    return host_call(ctx, rgba, raw_bytes(width, height), out, block_bytes(width, height, 16), out_bytes,
-                     [&](const uint8_t *di, uint8_t *dout, void *s) {
-                         return qtx_bc3_encode_device(ctx, params, di, width, height, dout, s);
-                     });
+    return host_encode(ctx, rgba, width, height, 16, out, out_bytes,
+                       [&](const uint8_t *di, int32_t band_h, uint8_t *dout, void *s) {
+                           return qtx_bc3_encode_device(ctx, params, di, width, band_h, dout, s);
+                       });
 }
 
 qtx_status qtx_bc4_encode(qtx_ctx *ctx, uint32_t channel, const uint8_t *rgba, int32_t width, int32_t height,

@@ -25,6 +25,13 @@ struct qtx_ctx {
     size_t out_cap = 0;
     uint32_t *d_flag = nullptr;  // result word of the alpha test
     qtx::rt::Stager stager;      // pinned staging of large host transfers (under mu)
+    // Row bands of the large host encodes (capi.cpp): a stream for their transfers
+    // and events for each band's upload and encode; made on the first such encode
+    // (under mu).  The runtime spreads a process's streams over a few hardware
+    // queues, so a context that never bands takes no queue for them.
+    static constexpr int kMaxBands = 8;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t band_up[kMaxBands] = {}, band_done[kMaxBands] = {};
     std::mutex mu;
     // Work-queue words of the queued BC1/BC3 lane kernel: kQueueSlots sets, each
     // lent to one launch at a time.  A set is free again once the event recorded

@@ -1,28 +1,52 @@
-"""GPU: the host entry points' staging (quicktex_amd/csrc/staging.cpp).
+"""GPU: the host entry points' staging and row bands (quicktex_amd/csrc/staging.cpp,
+quicktex_amd/csrc/capi.cpp).
 
-Host buffers of 4 MiB and more move through the context's two pinned 8 MiB
-chunks, a few threads copying one chunk while the DMA engine moves the other;
-smaller ones take the runtime's own path.  Checked here, byte for byte, against
-the device entry points on the same data: inputs just below and at the threshold,
-of two chunks with a short last one and of several chunks at ragged sizes, for
-encodes (the staged upload) and decodes (the staged download), and calls of
-different sizes in a row (a chunk's next use waits for its last DMA).
+Pageable host buffers of 8 MiB and more move through the context's pinned 4 MiB
+chunks, a few threads copying one chunk while the DMA engine moves the ones
+before it; smaller ones take the runtime's own path, and pinned ones go to the
+DMA engine directly.  BC1 and BC3 host encodes of 16 MiB of texels and more are
+cut into bands of block rows, each encoded as soon as its rows have landed, on
+two streams in turn.  Checked here, byte for byte, against the device entry
+points on the same data: inputs just below and at the staging threshold, of
+several chunks with a short last one, banded encodes at ragged sizes, of one
+block row's height per band and of wide, short textures, at cheap and expensive
+levels, from pageable and pinned memory; decodes (the staged download); an
+error raised by a band's launch; and calls of different sizes in a row (a
+chunk's next use waits for its last DMA).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 from quicktex_amd import synth
+from quicktex_amd.capi import QtxError, _ptr
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(1024, 1023), (1024, 1024), (2048, 1025), (2501, 3001)]  # 4 MiB - 4 KiB, 4 MiB, 2 chunks, 4 chunks
+# 8 MiB - 4 KiB, 8 MiB (two chunks), three chunks with a short last one, eight ragged chunks
+SIZES = [(1024, 2047), (1024, 2048), (2048, 1025), (2501, 3001)]
+# banded BC1/BC3 encodes: 16 MiB exactly, ragged both ways, wide and short (65
+# block rows), tall and narrow
+BANDED = [(2048, 2048), (2050, 2047), (16384, 257), (1024, 4100)]
+
+
+def _nb(w, h):
+    return ((w + 3) // 4) * ((h + 3) // 4)
 
 
 def _device_bc4(q, hip, img, channel):
     h, w = img.shape[:2]
-    nb = ((w + 3) // 4) * ((h + 3) // 4)
-    d_in, d_out = hip.Buffer.from_array(img), hip.Buffer(nb * 8)
+    d_in, d_out = hip.Buffer.from_array(img), hip.Buffer(_nb(w, h) * 8)
     q.bc4_encode_device(channel, d_in.ptr, w, h, d_out.ptr)
+    hip.synchronize()
+    return d_out.download().reshape(-1)
+
+
+def _device_colour(q, hip, img, p, bc3):
+    h, w = img.shape[:2]
+    d_in, d_out = hip.Buffer.from_array(img), hip.Buffer(_nb(w, h) * (16 if bc3 else 8))
+    (q.bc3_encode_device if bc3 else q.bc1_encode_device)(p, d_in.ptr, w, h, d_out.ptr)
     hip.synchronize()
     return d_out.download().reshape(-1)
 
@@ -34,40 +58,83 @@ def _device_bc1_decode(q, hip, blocks, w, h):
     return d_out.download().reshape(h, w, 4)
 
 
+def _first_bad(got, want, bs):
+    bad = np.nonzero((got.reshape(-1, bs) != want.reshape(-1, bs)).any(axis=1))[0]
+    return len(bad), bad[:4].tolist()
+
+
 @pytest.mark.parametrize('w,h', SIZES)
 def test_staged_encode(qtx, w, h):
     from quicktex_amd import hip
     img = synth.mixture_rgba(w, h, seed=900 + w, near_black=0.1)
     assert (qtx.bc4_encode(img, 1) == _device_bc4(qtx, hip, img, 1)).all()
     p = qtx.params(0, 4, 0)
-    d_in = hip.Buffer.from_array(img)
-    nb = ((w + 3) // 4) * ((h + 3) // 4)
-    d_out = hip.Buffer(nb * 8)
-    qtx.bc1_encode_device(p, d_in.ptr, w, h, d_out.ptr)
-    hip.synchronize()
-    assert (qtx.bc1_encode(img, p) == d_out.download().reshape(-1)).all()
+    assert (qtx.bc1_encode(img, p) == _device_colour(qtx, hip, img, p, False)).all()
 
 
 @pytest.mark.parametrize('w,h', SIZES)
 def test_staged_decode(qtx, w, h):
     from quicktex_amd import hip
     rng = np.random.default_rng(w * h)
-    nb = ((w + 3) // 4) * ((h + 3) // 4)
-    blocks = rng.integers(0, 256, nb * 8, dtype=np.uint8)
+    blocks = rng.integers(0, 256, _nb(w, h) * 8, dtype=np.uint8)
     got = qtx.bc1_decode(blocks, w, h, 0, True)
     assert (got == _device_bc1_decode(qtx, hip, blocks, w, h)).all()
 
 
+@pytest.mark.parametrize('w,h', BANDED)
+@pytest.mark.parametrize('level,mode,interp,bc3', [(18, 0x13, 0, False), (5, 4, 2, False), (10, 4, 3, True)])
+def test_banded_encode(qtx, w, h, level, mode, interp, bc3):
+    from quicktex_amd import hip
+    print('banded %dx%d level %d bc3 %d' % (w, h, level, bc3), flush=True)
+    img = synth.mixture_rgba(w, h, seed=0xBA4D + w + h, near_black=0.1)
+    p = qtx.params(level, mode, interp)
+    want = _device_colour(qtx, hip, img, p, bc3)
+    got = (qtx.bc3_encode if bc3 else qtx.bc1_encode)(img, p)
+    n, first = _first_bad(got, want, 16 if bc3 else 8)
+    assert n == 0, '%dx%d: %d blocks differ, first %s' % (w, h, n, first)
+
+
+def test_banded_encode_pinned(qtx):
+    """Pinned input and output: the bands' uploads and downloads go to the DMA engine directly."""
+    from quicktex_amd import hip
+    w, h = 2500, 2100
+    img = synth.mixture_rgba(w, h, seed=0xBA4E, near_black=0.1)
+    p = qtx.params(12, 4, 1)
+    want = _device_colour(qtx, hip, img, p, True)
+    src, dst = hip.Pinned(img.nbytes), hip.Pinned(want.nbytes)
+    src.array[:] = img.reshape(-1)
+    dst.array[:] = 0xA5
+    qtx.check(qtx.lib.qtx_bc3_encode(qtx.ctx, ctypes.byref(p), _ptr(src.array), w, h, _ptr(dst.array), dst.nbytes))
+    n, first = _first_bad(dst.array, want, 16)
+    assert n == 0, '%d blocks differ, first %s' % (n, first)
+
+
+def test_banded_encode_error_then_ok(qtx):
+    """A launch refused in the first band fails the call; the context works on."""
+    w, h = 2048, 2048
+    img = synth.mixture_rgba(w, h, seed=0xBA4F, near_black=0.1)
+    p = qtx.params(5, 4, 0)
+    p.interpolator = 9
+    with pytest.raises(QtxError):
+        qtx.bc1_encode(img, p)
+    from quicktex_amd import hip
+    p = qtx.params(5, 4, 0)
+    assert (qtx.bc1_encode(img, p) == _device_colour(qtx, hip, img, p, False)).all()
+
+
 def test_calls_in_a_row(qtx):
-    """Large, small and large again, encodes and decodes interleaved, repeated."""
+    """Large, small and large again, banded and plain encodes and decodes interleaved, repeated."""
     from quicktex_amd import hip
     big = synth.mixture_rgba(3000, 2000, seed=911, near_black=0.1)
     small = synth.mixture_rgba(300, 200, seed=912, near_black=0.1)
     want_big, want_small = _device_bc4(qtx, hip, big, 0), _device_bc4(qtx, hip, small, 0)
+    p = qtx.params(8, 4, 0)
+    want_colour = _device_colour(qtx, hip, big, p, False)
     rng = np.random.default_rng(913)
     blocks = rng.integers(0, 256, 750 * 500 * 8, dtype=np.uint8)
     want_dec = _device_bc1_decode(qtx, hip, blocks, 3000, 2000)
     for _ in range(3):
         assert (qtx.bc4_encode(big, 0) == want_big).all()
+        assert (qtx.bc1_encode(big, p) == want_colour).all()
         assert (qtx.bc1_decode(blocks, 3000, 2000, 0, True) == want_dec).all()
         assert (qtx.bc4_encode(small, 0) == want_small).all()
