@@ -303,12 +303,27 @@ __device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
 __device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
 
 // ErrorMode::Check2's scale of the projection on the axis (BC1Encoder.cpp:721-749):
-// 4 / (|axis|^2 + 1.25e-6).  It is below 4 unless the axis is zero, and then the
-// projection (p - c0) . axis is zero too; projections are at most 2 x 3 x 255^2 in
-// magnitude, so projection x f + 0.5 always converts to int in range, and the
-// truncating conversion needs no out-of-range case.
-__device__ __forceinline__ float check2_f(int a0, int a1, int a2) {
-    return 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
+// 4 / (|axis|^2 + 1.25e-6), positive; the projection's candidate index follows from
+// it through check2_cuts.
+__device__ __forceinline__ float check2_f(uint32_t axis2) { return 4.0f / ((float)axis2 + .00000125f); }
+
+// Check2's candidate index k = clamp(int(x * f + 0.5f), 1, 3) for the projection x =
+// (p - c0) . axis, as two cuts on x: k = 1 + (x >= t2) + (x >= t3).  x is an integer
+// of magnitude at most 3 x 255^2, exact as a float, and x * f + 0.5f, rounded at each
+// step, does not decrease as x grows (f > 0): k reaches 2 exactly from the least x
+// whose value reaches 2.0 (t2), and 3 from the least reaching 3.0 (t3).  For every
+// |axis|^2 = A in [0, 3 x 255^2] those are ceil(3A / 8) or the next integer, and
+// ceil(5A / 8) or the next (found by evaluating every A), so one evaluation of the
+// rounded expression at the first decides each.  Returned as cuts on d3 - d0 =
+// x + c0a, the form the selector loops hold.
+struct Check2Cuts {
+    int t2, t3;
+};
+__device__ __forceinline__ Check2Cuts check2_cuts(uint32_t axis2, float f, int c0a) {
+    const int c2 = (int)((3u * axis2 + 7u) >> 3), c3 = (int)((5u * axis2 + 7u) >> 3);
+    const int t2 = (float)c2 * f + 0.5f >= 2.0f ? c2 : c2 + 1;
+    const int t3 = (float)c3 * f + 0.5f >= 3.0f ? c3 : c3 + 1;
+    return Check2Cuts{t2 + c0a, t3 + c0a};
 }
 
 // Texel-parallel, with selectors: returns (error, selectors).
@@ -332,11 +347,12 @@ __device__ __forceinline__ uint2 eval_px(const Ctx &cx, int M, int error_mode, u
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
-        const int da = (d3 - d0) - ((int)dot4(c[0], c[3]) - (int)dot4(c[0], c[0]));  // (p - c0) . axis
-        const int k = clampi(cvt_trunc((float)da * f + 0.5f), 1, 3);
-        const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);  // candidate k-1
-        const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);  // candidate k
+        const uint32_t axis2 = (uint32_t)(a0 * a0 + a1 * a1 + a2 * a2);
+        const Check2Cuts cut = check2_cuts(axis2, check2_f(axis2), (int)dot4(c[0], c[3]) - (int)dot4(c[0], c[0]));
+        const bool ge2 = d3 - d0 >= cut.t2, ge3 = d3 - d0 >= cut.t3;
+        const int k = 1 + (int)ge2 + (int)ge3;
+        const int ua = ge2 ? (ge3 ? u2 : u1) : u0;  // candidate k-1
+        const int ub = ge2 ? (ge3 ? u3 : u2) : u1;  // candidate k
         s = (uint32_t)k;
         bu = ub;
         if (ua == ub) {
@@ -400,8 +416,8 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = 4.0f / ((float)(uint32_t)(a0 * a0 + a1 * a1 + a2 * a2) + .00000125f);
-        const int c0a = (int)dot4(c[0], c[3]) - cc0;
+        const uint32_t axis2 = (uint32_t)(a0 * a0 + a1 * a1 + a2 * a2);
+        const Check2Cuts cut = check2_cuts(axis2, check2_f(axis2), (int)dot4(c[0], c[3]) - cc0);
         for (int q = 0; q < 4; q++) {
             const uint4 t = tex4(q);
 #pragma unroll
@@ -409,11 +425,11 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
                 const uint32_t p = pick(t, k);
                 const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
                 const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-                const int s = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+                const bool ge2 = d3 - d0 >= cut.t2, ge3 = d3 - d0 >= cut.t3;
                 const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
                 const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
-                const int ua = s == 1 ? u0 : (s == 2 ? u1 : u2);
-                const int ub = s == 1 ? u1 : (s == 2 ? u2 : u3);
+                const int ua = ge2 ? (ge3 ? u2 : u1) : u0;
+                const int ub = ge2 ? (ge3 ? u3 : u2) : u1;
                 tot += max(ua, ub);
             }
         }

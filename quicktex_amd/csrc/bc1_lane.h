@@ -474,8 +474,8 @@ __device__ __forceinline__ uint32_t find_selectors_c(const uint32_t (&px)[16], c
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = check2_f(a0, a1, a2);
-        const int c0a = (int)dot4(c[0], c[3]) - cc0;
+        const uint32_t axis2 = (uint32_t)(a0 * a0 + a1 * a1 + a2 * a2);
+        const Check2Cuts cut = check2_cuts(axis2, check2_f(axis2), (int)dot4(c[0], c[3]) - cc0);
         const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
         const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
 #pragma unroll
@@ -483,11 +483,12 @@ __device__ __forceinline__ uint32_t find_selectors_c(const uint32_t (&px)[16], c
             const uint32_t p = px[i];
             const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
             const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-            const int k = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+            const bool ge2 = d3 - d0 >= cut.t2, ge3 = d3 - d0 >= cut.t3;
+            const int k = 1 + (int)ge2 + (int)ge3;
             const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
             const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
-            const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);  // candidate k-1
-            const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);  // candidate k
+            const int ua = ge2 ? (ge3 ? u2 : u1) : u0;  // candidate k-1
+            const int ub = ge2 ? (ge3 ? u3 : u2) : u1;  // candidate k
             uint32_t s = (uint32_t)k;
             if (ua == ub) {
                 if (k == 1) s = 0;
@@ -554,8 +555,8 @@ __device__ __forceinline__ uint32_t trial_error_c(const uint32_t (&px)[16], cons
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
         const int a1 = (int)chn(c[3], 1) - (int)chn(c[0], 1);
         const int a2 = (int)chn(c[3], 2) - (int)chn(c[0], 2);
-        const float f = check2_f(a0, a1, a2);
-        const int c0a = (int)dot4(c[0], c[3]) - cc0;
+        const uint32_t axis2 = (uint32_t)(a0 * a0 + a1 * a1 + a2 * a2);
+        const Check2Cuts cut = check2_cuts(axis2, check2_f(axis2), (int)dot4(c[0], c[3]) - cc0);
         const uint32_t n0 = neg_opaque((uint32_t)cc0), n1 = neg_opaque((uint32_t)cc1);
         const uint32_t n2 = neg_opaque((uint32_t)cc2), n3 = neg_opaque((uint32_t)cc3);
 #pragma unroll
@@ -563,11 +564,11 @@ __device__ __forceinline__ uint32_t trial_error_c(const uint32_t (&px)[16], cons
             const uint32_t p = px[i];
             const int d0 = (int)dot4(p, c[0]), d1 = (int)dot4(p, c[1]);
             const int d2 = (int)dot4(p, c[2]), d3 = (int)dot4(p, c[3]);
-            const int k = clampi((int)((float)((d3 - d0) - c0a) * f + 0.5f), 1, 3);  // in range: see check2_f
+            const bool ge2 = d3 - d0 >= cut.t2, ge3 = d3 - d0 >= cut.t3;
             const int u0 = twice_plus((uint32_t)d0, n0), u1 = twice_plus((uint32_t)d1, n1);
             const int u2 = twice_plus((uint32_t)d2, n2), u3 = twice_plus((uint32_t)d3, n3);
-            const int ua = k == 1 ? u0 : (k == 2 ? u1 : u2);
-            const int ub = k == 1 ? u1 : (k == 2 ? u2 : u3);
+            const int ua = ge2 ? (ge3 ? u2 : u1) : u0;
+            const int ub = ge2 ? (ge3 ? u3 : u2) : u1;
             tot += max(ua, ub);
         }
         return bk.pp - (uint32_t)tot;
