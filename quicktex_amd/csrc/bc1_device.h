@@ -40,6 +40,9 @@ constexpr int G = 16;  // lanes per block
 __device__ __forceinline__ uint32_t chn(uint32_t c, int i) { return (c >> (8 * i)) & 0xFFu; }
 __device__ __forceinline__ uint32_t rgb(uint32_t r, uint32_t g, uint32_t b) { return r | (g << 8) | (b << 16); }
 __device__ __forceinline__ uint32_t dot4(uint32_t a, uint32_t b) { return __builtin_amdgcn_udot4(a, b, 0u, false); }
+__device__ __forceinline__ uint32_t dot4acc(uint32_t a, uint32_t b, uint32_t acc) {
+    return __builtin_amdgcn_udot4(a, b, acc, false);
+}
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ uint32_t s5to8(uint32_t v) { return (v << 3) | (v >> 2); }
 __device__ __forceinline__ uint32_t s6to8(uint32_t v) { return (v << 2) | (v >> 4); }
@@ -301,6 +304,22 @@ __device__ __forceinline__ uint32_t neg_opaque(uint32_t v) {
     return n;
 }
 __device__ __forceinline__ int twice_plus(uint32_t d, uint32_t n) { return (int)((d << 1) + n); }
+
+// The Faster (and None) selector of a texel (BC1Encoder.cpp:695-720): s = 3 - [dot <=
+// t0] - [dot < t1] - [dot < t2] for dot = 2 x, x = (d3 - d0) the texel's projection on
+// the axis and t_j the sums of neighbouring palette projections.  As cuts on x:
+// [2x <= t] = [x <= t >> 1] and [2x < t] = [x <= (t - 1) >> 1] (arithmetic shifts round
+// down).  s counts the cuts below x, which sorting them does not change, and with them
+// sorted (f0 <= f1 <= f2) s = 0 up to f0, 1 up to f1, 2 up to f2 and 3 above: the
+// palette entry follows from three compares without forming s.
+struct FasterCuts {
+    int f0, f1, f2;
+};
+__device__ __forceinline__ FasterCuts faster_cuts(int t0, int t1, int t2) {
+    const int e0 = t0 >> 1, e1 = (t1 - 1) >> 1, e2 = (t2 - 1) >> 1;
+    const int lo = min(e0, e1), hi = max(e0, e1);
+    return FasterCuts{min(lo, e2), max(lo, min(hi, e2)), max(hi, e2)};
+}
 
 // ErrorMode::Check2's scale of the projection on the axis (BC1Encoder.cpp:721-749):
 // 4 / (|axis|^2 + 1.25e-6), positive; the projection's candidate index follows from

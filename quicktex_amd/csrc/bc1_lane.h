@@ -455,20 +455,20 @@ __device__ __forceinline__ uint32_t find_selectors_c(const uint32_t (&px)[16], c
         const int k1 = (int)dot4(c[3], c[1]) - (int)dot4(c[0], c[1]);
         const int k2 = (int)dot4(c[3], c[2]) - (int)dot4(c[0], c[2]);
         const int k3 = cc3 - (int)dot4(c[0], c[3]);
-        const int t0 = k0 + k1, t1 = k1 + k2, t2 = k2 + k3;
+        const FasterCuts cut = faster_cuts(k0 + k1, k1 + k2, k2 + k3);
+        uint32_t pc = 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t p = px[i];
-            const int d0 = (int)dot4(p, c[0]), d3 = (int)dot4(p, c[3]);
-            const int dot = 2 * (d3 - d0);
-            const uint32_t s = 3u - (uint32_t)((dot <= t0) + (dot < t1) + (dot < t2));
-            const uint32_t cs = s == 0 ? c[0] : (s == 1 ? c[1] : (s == 2 ? c[2] : c[3]));
-            const int ccs = s == 0 ? cc0 : (s == 1 ? cc1 : (s == 2 ? cc2 : cc3));
-            tot += 2 * (int)dot4(p, cs) - ccs;
-            s_all |= s << (2 * i);
+            const int x = (int)dot4(p, c[3]) - (int)dot4(p, c[0]);
+            const bool b0 = x <= cut.f0, b1 = x <= cut.f1, b2 = x <= cut.f2;
+            const uint32_t cs = b0 ? c[0] : (b1 ? c[1] : (b2 ? c[2] : c[3]));
+            tot += b0 ? cc0 : (b1 ? cc1 : (b2 ? cc2 : cc3));
+            pc = dot4acc(p, cs, pc);
+            s_all |= (3u - (uint32_t)b0 - (uint32_t)b1 - (uint32_t)b2) << (2 * i);
         }
         sel = s_all;
-        return error_mode == QTX_ERROR_NONE ? 0u : bk.pp - (uint32_t)tot;
+        return error_mode == QTX_ERROR_NONE ? 0u : bk.pp - (2u * pc - (uint32_t)tot);
     }
     if (error_mode == QTX_ERROR_CHECK2) {
         const int a0 = (int)chn(c[3], 0) - (int)chn(c[0], 0);
@@ -572,6 +572,24 @@ __device__ __forceinline__ uint32_t trial_error_c(const uint32_t (&px)[16], cons
             tot += max(ua, ub);
         }
         return bk.pp - (uint32_t)tot;
+    }
+    if (error_mode == QTX_ERROR_FASTER) {
+        const int k0 = (int)dot4(c[3], c[0]) - cc0;
+        const int k1 = (int)dot4(c[3], c[1]) - (int)dot4(c[0], c[1]);
+        const int k2 = (int)dot4(c[3], c[2]) - (int)dot4(c[0], c[2]);
+        const int k3 = cc3 - (int)dot4(c[0], c[3]);
+        const FasterCuts cut = faster_cuts(k0 + k1, k1 + k2, k2 + k3);
+        uint32_t pc = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t p = px[i];
+            const int x = (int)dot4(p, c[3]) - (int)dot4(p, c[0]);
+            const bool b0 = x <= cut.f0, b1 = x <= cut.f1, b2 = x <= cut.f2;
+            const uint32_t cs = b0 ? c[0] : (b1 ? c[1] : (b2 ? c[2] : c[3]));
+            tot += b0 ? cc0 : (b1 ? cc1 : (b2 ? cc2 : cc3));
+            pc = dot4acc(p, cs, pc);
+        }
+        return bk.pp - (2u * pc - (uint32_t)tot);
     }
     uint32_t sel;
     return find_selectors_c<INTERP, PAL>(px, bk, M, error_mode, c, sel);

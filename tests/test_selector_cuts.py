@@ -1,5 +1,6 @@
-"""CPU: the integer cuts that give Check2's candidate index in the lane and
-16-lane encoders (quicktex_amd/csrc/bc1_device.h, check2_cuts).
+"""CPU: the integer cuts that give the selector searches' candidate index in the
+lane and 16-lane encoders (quicktex_amd/csrc/bc1_device.h, check2_cuts and
+faster_cuts).
 
 The reference's Check2 selector search (quicktex/s3tc/bc1/BC1Encoder.cpp:721-749)
 takes, per texel, k = clamp(int(x * f + 0.5f), 1, 3) for the integer projection
@@ -10,6 +11,12 @@ expression at c2 decides, and t3 likewise from ceil(5A / 8).  Here the rule is
 checked for every A an axis of 8-bit colours can have, against the float
 expression itself at and around each cut, and over random and extreme
 projections; a cut one off (or the rule without its correction) fails.
+
+The Faster (and None) search (BC1Encoder.cpp:695-720) takes s = 3 - [2x <= t0] -
+[2x < t1] - [2x < t2].  The encoders compare x with the sorted cuts t0 >> 1,
+(t1 - 1) >> 1, (t2 - 1) >> 1 and pick the palette entry from three nested
+compares; checked against the reference's form over random, tied, unsorted and
+extreme thresholds.
 """
 import numpy as np
 
@@ -69,3 +76,33 @@ def test_correction_is_needed():
     c2, c3 = (3 * A + 7) >> 3, (5 * A + 7) >> 3
     assert (t2 != c2).sum() > 100 and (t3 != c3).sum() > 100
     assert ((A[t2 != c2] % 8) == 0).all()
+
+
+def _faster_ref(x, t0, t1, t2):
+    d = 2 * x
+    return 3 - ((d <= t0).astype(np.int64) + (d < t1) + (d < t2))
+
+
+def _faster_cuts(x, t0, t1, t2):
+    e = np.stack([t0 >> 1, (t1 - 1) >> 1, (t2 - 1) >> 1])
+    f0, f1, f2 = np.sort(e, axis=0)
+    return np.where(x <= f0, 0, np.where(x <= f1, 1, np.where(x <= f2, 2, 3)))
+
+
+def test_faster_cuts():
+    rng = np.random.default_rng(0xFA57)
+    n = 400000
+    lim = 2 * AMAX
+    t = rng.integers(-lim, lim + 1, (3, n))
+    x = rng.integers(-AMAX, AMAX + 1, n)
+    # ties between thresholds, thresholds out of order, x at and next to each threshold
+    t[1, :n // 8] = t[0, :n // 8]
+    t[2, n // 8:n // 4] = t[1, n // 8:n // 4]
+    j = rng.integers(0, 3, n)
+    near = (t[j, np.arange(n)] >> 1) + rng.integers(-1, 2, n)
+    x[n // 4:] = near[n // 4:]
+    assert np.array_equal(_faster_cuts(x, *t), _faster_ref(x, *t))
+    # off by one in a cut's rounding changes some selectors
+    e_bad = (t[1] >> 1)
+    bad = np.where(x <= np.minimum(t[0] >> 1, np.minimum(e_bad, (t[2] - 1) >> 1)), 0, 1)
+    assert not np.array_equal(bad == 0, _faster_ref(x, *t) == 0)
