@@ -386,10 +386,11 @@ __device__ __forceinline__ uint2 eval_px(const Ctx &cx, int M, int error_mode, u
         const int k1 = (int)dot4(c[3], c[1]) - (int)dot4(c[0], c[1]);
         const int k2 = (int)dot4(c[3], c[2]) - (int)dot4(c[0], c[2]);
         const int k3 = (int)dot4(c[3], c[3]) - (int)dot4(c[0], c[3]);
-        const int dot = 2 * (d3 - d0);
-        const int level = (dot <= k0 + k1) + (dot < k1 + k2) + (dot < k2 + k3);
-        s = (uint32_t)(3 - level);
-        bu = s == 0 ? u0 : (s == 1 ? u1 : (s == 2 ? u2 : u3));
+        const FasterCuts cut = faster_cuts(k0 + k1, k1 + k2, k2 + k3);
+        const int x = d3 - d0;
+        const bool b0 = x <= cut.f0, b1 = x <= cut.f1, b2 = x <= cut.f2;
+        s = 3u - (uint32_t)b0 - (uint32_t)b1 - (uint32_t)b2;
+        bu = b0 ? u0 : (b1 ? u1 : (b2 ? u2 : u3));
     }
     const int tot = (int)gsum((uint32_t)bu);
     const uint32_t sel = gor(s << (2 * glane()));
@@ -459,21 +460,21 @@ __device__ __forceinline__ uint32_t trial_err(const Ctx &cx, int M, int error_mo
         const int k1 = (int)dot4(c[3], c[1]) - (int)dot4(c[0], c[1]);
         const int k2 = (int)dot4(c[3], c[2]) - (int)dot4(c[0], c[2]);
         const int k3 = cc3 - (int)dot4(c[0], c[3]);
-        const int t0 = k0 + k1, t1 = k1 + k2, t2 = k2 + k3;
+        const FasterCuts cut = faster_cuts(k0 + k1, k1 + k2, k2 + k3);
+        uint32_t pc = 0;
         for (int q = 0; q < 4; q++) {
             const uint4 t = tex4(q);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint32_t p = pick(t, k);
-                const int d0 = (int)dot4(p, c[0]), d3 = (int)dot4(p, c[3]);
-                const int dot = 2 * (d3 - d0);
-                const int s = 3 - ((dot <= t0) + (dot < t1) + (dot < t2));
-                const uint32_t cs = s == 0 ? c[0] : (s == 1 ? c[1] : (s == 2 ? c[2] : c[3]));
-                const int ccs = s == 0 ? cc0 : (s == 1 ? cc1 : (s == 2 ? cc2 : cc3));
-                tot += 2 * (int)dot4(p, cs) - ccs;
+                const int x = (int)dot4(p, c[3]) - (int)dot4(p, c[0]);
+                const bool b0 = x <= cut.f0, b1 = x <= cut.f1, b2 = x <= cut.f2;
+                const uint32_t cs = b0 ? c[0] : (b1 ? c[1] : (b2 ? c[2] : c[3]));
+                tot += b0 ? cc0 : (b1 ? cc1 : (b2 ? cc2 : cc3));
+                pc = dot4acc(p, cs, pc);
             }
         }
-        return (uint32_t)(cx.pp - tot);
+        return (uint32_t)(cx.pp - (int)(2u * pc) + tot);
     }
     return 0u;
 }
