@@ -11,8 +11,9 @@ points on the same data: inputs just below and at the staging threshold, of
 several chunks with a short last one, banded encodes at ragged sizes, of one
 block row's height per band and of wide, short textures, at cheap and expensive
 levels, from pageable and pinned memory; decodes (the staged download); an
-error raised by a band's launch; and calls of different sizes in a row (a
-chunk's next use waits for its last DMA).
+error raised by a band's launch; calls of different sizes in a row (a chunk's
+next use waits for its last DMA); and threads sharing a context beside threads
+with contexts of their own.
 """
 import ctypes
 
@@ -138,3 +139,49 @@ def test_calls_in_a_row(qtx):
         assert (qtx.bc1_encode(big, p) == want_colour).all()
         assert (qtx.bc1_decode(blocks, 3000, 2000, 0, True) == want_dec).all()
         assert (qtx.bc4_encode(small, 0) == want_small).all()
+
+
+def test_threads_and_contexts(qtx):
+    """Four threads share one context (its host lock serialises the staging and the
+    bands) while two more each use a context of their own, all with large banded
+    encodes, staged decodes and small calls; every result equals the device path's."""
+    import threading
+
+    from quicktex_amd import hip
+    from tests._qtx import Qtx
+    imgs = [synth.mixture_rgba(2100 + 60 * k, 2050, seed=0x7A + k, near_black=0.1) for k in range(3)]
+    small = synth.mixture_rgba(200, 120, seed=0x7F, near_black=0.1)
+    p = qtx.params(9, 4, 1)
+    want = [_device_colour(qtx, hip, im, p, False) for im in imgs]
+    want_small = _device_colour(qtx, hip, small, p, False)
+    rng = np.random.default_rng(0x7B)
+    blocks = rng.integers(0, 256, 600 * 520 * 8, dtype=np.uint8)
+    want_dec = _device_bc1_decode(qtx, hip, blocks, 2400, 2080)
+    errors = []
+
+    def work(q, seed):
+        try:
+            order = np.random.default_rng(seed).permutation(6)
+            for i in order:
+                if i < 3:
+                    assert (q.bc1_encode(imgs[i], p) == want[i]).all(), ('encode', i)
+                elif i == 3:
+                    assert (q.bc1_decode(blocks, 2400, 2080, 0, True) == want_dec).all(), 'decode'
+                else:
+                    assert (q.bc1_encode(small, p) == want_small).all(), 'small'
+        except Exception as e:  # reported on the main thread
+            errors.append(repr(e))
+
+    own = [Qtx(), Qtx()]
+    try:
+        threads = [threading.Thread(target=work, args=(qtx, 100 + k)) for k in range(4)]
+        threads += [threading.Thread(target=work, args=(q, 200 + k)) for k, q in enumerate(own)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(120)
+        assert not any(t.is_alive() for t in threads), 'a thread did not finish'
+    finally:
+        for q in own:
+            q.close()
+    assert not errors, errors
