@@ -178,6 +178,44 @@ hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t 
     return launch_light_batch(dj, (int)g.order.size(), g.nunits, g.grid, s);
 }
 
+qtx_status run_once(qtx_ctx *ctx, const std::vector<qtx_job> &jobs, hipStream_t s,
+                    std::vector<BatchJob> &host_table) {
+    std::vector<Prepared> prep(jobs.size());
+    qtx_status st;
+    for (size_t i = 0; i < jobs.size(); i++)
+        if ((st = prepare(jobs[i], i, prep[i])) != QTX_OK) return st;
+    Plan pl;
+    if ((st = plan(prep, qtx_batch_tuning{kMaxColourWgPerCu, 256}, ctx->cu_count, ctx->bc1_path, pl)) != QTX_OK)
+        return st;
+    constexpr size_t kQueueBytes = 256;  // two words per kernel group
+    static_assert(2 * kGroups * sizeof(uint32_t) <= kQueueBytes, "queue words fit");
+    const size_t need = kQueueBytes + pl.host.size() * sizeof(BatchJob);
+    if (need > ctx->once_cap) {
+        // hipFree waits for the device: the last run's kernels are done with the table
+        if (ctx->d_once) (void)hipFree(ctx->d_once);
+        ctx->d_once = nullptr;
+        ctx->once_cap = 0;
+        const size_t cap = std::max(need, (size_t)64 << 10);
+        QTX_HIP(hipMalloc(&ctx->d_once, cap));
+        const hipError_t e = hipMemset(ctx->d_once, 0, kQueueBytes);
+        if (e != hipSuccess) {
+            (void)hipFree(ctx->d_once);
+            ctx->d_once = nullptr;
+            return hip_fail(e, "batch queue words");
+        }
+        ctx->once_cap = cap;
+    }
+    uint32_t *queues = reinterpret_cast<uint32_t *>(ctx->d_once);
+    auto *table = reinterpret_cast<BatchJob *>(ctx->d_once + kQueueBytes);
+    // the table is copied in stream order, after the last run's kernels on s, from
+    // the caller's vector
+    host_table = std::move(pl.host);
+    QTX_HIP(hipMemcpyAsync(table, host_table.data(), host_table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s));
+    for (size_t g = 0; g < pl.groups.size(); g++)
+        QTX_HIP(enqueue_group(pl.groups[g], table, queues + 2 * g, ctx->d_tables, s));
+    return QTX_OK;
+}
+
 }  // namespace bt
 }  // namespace qtx
 
@@ -430,26 +468,27 @@ qtx_status qtx_batch_run_host(qtx_ctx *ctx, const qtx_job *jobs, const size_t *d
     std::lock_guard<std::mutex> lk(ctx->mu);
     if ((st = ensure(ctx->d_in, ctx->in_cap, total)) != QTX_OK) return st;
     std::vector<qtx_job> dev(jobs, jobs + njobs);
+    // the sources lie in one range of the device buffer and the destinations in
+    // another: one staged transfer each way (parts only read on the way up)
+    std::vector<rt::HostPart> up(njobs), down(njobs);
     for (size_t i = 0; i < njobs; i++) {
-        QTX_HIP(ctx->stager.upload(ctx->d_in + src_off[i], static_cast<const uint8_t *>(jobs[i].src),
-                                   prep[i].src_bytes, ctx->stream));
+        up[i] = {const_cast<uint8_t *>(static_cast<const uint8_t *>(jobs[i].src)), src_off[i], prep[i].src_bytes};
+        down[i] = {static_cast<uint8_t *>(jobs[i].dst), dst_off[i] - dst_off[0], prep[i].dst_bytes};
         dev[i].src = ctx->d_in + src_off[i];
         dev[i].dst = ctx->d_in + dst_off[i];
     }
-    qtx_batch *b = nullptr;
-    if ((st = qtx_batch_create(ctx, dev.data(), njobs, &b)) != QTX_OK) {
-        (void)hipStreamSynchronize(ctx->stream);
-        return st;
-    }
-    st = qtx_batch_launch(b, ctx->stream);
-    for (size_t i = 0; i < njobs && st == QTX_OK; i++) {
-        const hipError_t e = ctx->stager.download(static_cast<uint8_t *>(jobs[i].dst), ctx->d_in + dst_off[i],
-                                                  prep[i].dst_bytes, ctx->stream);
+    QTX_HIP(ctx->stager.upload_parts(ctx->d_in, up.data(), njobs, src_off[njobs - 1] + prep[njobs - 1].src_bytes,
+                                     ctx->stream));
+    std::vector<BatchJob> table;  // read by the copy to the device until the stream is synchronised
+    st = run_once(ctx, dev, ctx->stream, table);
+    if (st == QTX_OK) {
+        const hipError_t e = ctx->stager.download_parts(down.data(), njobs, ctx->d_in + dst_off[0],
+                                                        dst_off[njobs - 1] - dst_off[0] + prep[njobs - 1].dst_bytes,
+                                                        ctx->stream);
         if (e != hipSuccess) st = hip_fail(e, "download");
     }
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
-    qtx_batch_destroy(b);
     return st;
 }
 

@@ -51,6 +51,13 @@ qtx_status plan(const std::vector<Prepared> &jobs, const qtx_batch_tuning &t, in
 // queue words (zero before the first launch; the kernels leave them zero).
 hipError_t enqueue_group(const Plan::Group &g, const BatchJob *d_jobs, uint32_t *d_queue,
                          const DeviceTables *d_tables, hipStream_t s);
+// Enqueues jobs (device buffers) as one batch on s with qtx_batch_create's default
+// tuning: the plan's group kernels one after the other over the context's d_once
+// buffer -- no graph, streams or allocations per call, unlike a qtx_batch.  Under
+// ctx->mu, with every run of the context on s.  host_table receives the job table
+// the copy to the device reads: the caller keeps it until s is synchronised.
+qtx_status run_once(qtx_ctx *ctx, const std::vector<qtx_job> &jobs, hipStream_t s,
+                    std::vector<BatchJob> &host_table);
 
 }  // namespace bt
 }  // namespace qtx

@@ -326,6 +326,7 @@ qtx_status qtx_ctx_destroy(qtx_ctx *ctx) {
     if (ctx->d_in) (void)hipFree(ctx->d_in);
     if (ctx->d_out) (void)hipFree(ctx->d_out);
     if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    if (ctx->d_once) (void)hipFree(ctx->d_once);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     for (hipEvent_t &ev : ctx->queue_done) {
         if (!ev) continue;

@@ -28,6 +28,7 @@ qtx_status chain_plan(int32_t w, int32_t h, int32_t levels, qtx::MipPlan &p, siz
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+
 }  // namespace
 
 extern "C" {
@@ -142,19 +143,17 @@ qtx_status encode_chain(qtx_ctx *ctx, const qtx_job &tmpl, const uint8_t *rgba, 
         jobs[m].src = d + img_off[m];
         jobs[m].dst = d + blk_off[m];
     }
-    qtx_batch *b = nullptr;
-    if ((st = qtx_batch_create(ctx, jobs.data(), (size_t)levels, &b)) != QTX_OK) {
-        (void)hipStreamSynchronize(ctx->stream);
-        return st;
-    }
-    st = qtx_batch_launch(b, ctx->stream);
-    for (int m = 0; m < levels && st == QTX_OK; m++) {
-        const hipError_t e = ctx->stager.download(out_blocks[m], d + blk_off[m], prep[m].dst_bytes, ctx->stream);
+    std::vector<qtx::BatchJob> table;  // read by the copy to the device until the stream is synchronised
+    st = qtx::bt::run_once(ctx, jobs, ctx->stream, table);
+    if (st == QTX_OK) {  // the levels' blocks lie in one range: one staged transfer
+        std::vector<qtx::rt::HostPart> parts(levels);
+        for (int m = 0; m < levels; m++) parts[m] = {out_blocks[m], blk_off[m] - blk_off[0], prep[m].dst_bytes};
+        const size_t n = blk_off[levels - 1] - blk_off[0] + prep[levels - 1].dst_bytes;
+        const hipError_t e = ctx->stager.download_parts(parts.data(), parts.size(), d + blk_off[0], n, ctx->stream);
         if (e != hipSuccess) st = hip_fail(e, "download");
     }
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     if (st == QTX_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
-    qtx_batch_destroy(b);
     return st;
 }
 

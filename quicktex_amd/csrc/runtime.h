@@ -32,6 +32,11 @@ struct qtx_ctx {
     static constexpr int kMaxBands = 8;
     hipStream_t copy_stream = nullptr;
     hipEvent_t band_up[kMaxBands] = {}, band_done[kMaxBands] = {};
+    // The batches the host entry points run once (batch.cpp, bt::run_once): the
+    // queue words of their kernel groups (zero between runs: the kernels leave them
+    // so), then the job table; grown on demand (under mu).
+    uint8_t *d_once = nullptr;
+    size_t once_cap = 0;
     std::mutex mu;
     // Work-queue words of the queued BC1/BC3 lane kernel: kQueueSlots sets, each
     // lent to one launch at a time.  A set is free again once the event recorded

@@ -159,5 +159,83 @@ hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStr
     return hipSuccess;
 }
 
+namespace {
+// Copies between a chunk [c0, c0 + len) of the range and the parts it overlaps:
+// to the parts (out) or from them into the chunk.
+void copy_pieces(CopyPool *pool, uint8_t *chunk, size_t c0, size_t len, const HostPart *parts, size_t nparts,
+                 size_t &first, bool out) {
+    while (first < nparts && parts[first].off + parts[first].len <= c0) first++;
+    for (size_t i = first; i < nparts && parts[i].off < c0 + len; i++) {
+        const size_t b = std::max(parts[i].off, c0), e = std::min(parts[i].off + parts[i].len, c0 + len);
+        if (e <= b) continue;
+        uint8_t *host = parts[i].host + (b - parts[i].off), *slot = chunk + (b - c0);
+        uint8_t *dst = out ? host : slot;
+        const uint8_t *src = out ? slot : host;
+        if (e - b >= ((size_t)256 << 10)) pool->copy(dst, src, e - b);
+        else std::memcpy(dst, src, e - b);
+    }
+}
+
+bool all_pinned(const HostPart *parts, size_t nparts) {
+    for (size_t i = 0; i < nparts; i++)
+        if (!host_pinned(parts[i].host)) return false;
+    return true;
+}
+}  // namespace
+
+hipError_t Stager::upload_parts(uint8_t *d_dst, const HostPart *parts, size_t nparts, size_t n, hipStream_t s) {
+    if (n < kPartsMinBytes || all_pinned(parts, nparts) || init() != hipSuccess) {
+        for (size_t i = 0; i < nparts; i++) {
+            const hipError_t e = upload(d_dst + parts[i].off, parts[i].host, parts[i].len, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    size_t first = 0;
+    for (size_t off = 0; off < n; off += kChunk) {
+        const size_t len = std::min(kChunk, n - off);
+        const unsigned k = next_++ % kSlots;
+        uint8_t *slot = pin_ + k * kChunk;
+        hipError_t e = hipEventSynchronize(ev_[k]);
+        if (e != hipSuccess) return e;
+        copy_pieces(pool_, slot, off, len, parts, nparts, first, false);
+        if ((e = hipMemcpyAsync(d_dst + off, slot, len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_[k], s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t Stager::download_parts(const HostPart *parts, size_t nparts, const uint8_t *d_src, size_t n, hipStream_t s) {
+    if (n < kPartsMinBytes || all_pinned(parts, nparts) || init() != hipSuccess) {
+        for (size_t i = 0; i < nparts; i++) {
+            const hipError_t e = download(parts[i].host, d_src + parts[i].off, parts[i].len, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    hipError_t e;
+    const size_t chunks = (n + kChunk - 1) / kChunk;
+    const unsigned first_slot = next_;
+    next_ += (unsigned)chunks;
+    auto slot = [&](size_t k) { return (unsigned)(first_slot + k) % kSlots; };
+    auto enqueue = [&](size_t k) -> hipError_t {
+        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        hipError_t r = hipEventSynchronize(ev_[slot(k)]);
+        if (r == hipSuccess) r = hipMemcpyAsync(pin_ + slot(k) * kChunk, d_src + off, len, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipEventRecord(ev_[slot(k)], s);
+        return r;
+    };
+    for (size_t k = 0; k < chunks && k + 1 < (size_t)kSlots; k++)
+        if ((e = enqueue(k)) != hipSuccess) return e;
+    size_t first = 0;
+    for (size_t k = 0; k < chunks; k++) {
+        if ((e = hipEventSynchronize(ev_[slot(k)])) != hipSuccess) return e;
+        if (k + kSlots - 1 < chunks && (e = enqueue(k + kSlots - 1)) != hipSuccess) return e;
+        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        copy_pieces(pool_, pin_ + slot(k) * kChunk, off, len, parts, nparts, first, true);
+    }
+    return hipSuccess;
+}
+
 }  // namespace rt
 }  // namespace qtx

@@ -217,3 +217,39 @@ def test_batch_errors(qtx, stream):
             b.tuning = t
     b.close()
     assert ctypes.sizeof(type(good)) == 96
+
+
+def test_batch_run_host_staged(qtx):
+    """qtx_batch_run_host with sources and destinations past the staging threshold:
+    large and tiny jobs interleaved, so that parts straddle the 8 MiB chunks both
+    ways; every output equals the plain host entry point's."""
+    from quicktex_amd import synth
+    rng = np.random.default_rng(0x57A6)
+    p = qtx.params(6, 4, 2)
+    specs = [(OP_ENCODE, FMT_BC1, 1500, 1100), (OP_ENCODE, FMT_BC3, 17, 9), (OP_DECODE, FMT_BC4, 2048, 1100),
+             (OP_ENCODE, FMT_BC4, 2100, 1300), (OP_DECODE, FMT_BC1, 5, 3), (OP_ENCODE, FMT_BC3, 1800, 1500),
+             (OP_DECODE, FMT_BC3, 1200, 1024)]
+    jobs, want = [], []
+    for k, (op, fmt, w, h) in enumerate(specs):
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        bpb = 16 if fmt == FMT_BC3 else 8
+        if op == OP_ENCODE:
+            src = synth.mixture_rgba(w, h, seed=0x57A0 + k, near_black=0.1)
+            out = np.zeros(nb * bpb, np.uint8)
+            ref = {FMT_BC1: lambda: qtx.bc1_encode(src, p), FMT_BC3: lambda: qtx.bc3_encode(src, p),
+                   FMT_BC4: lambda: qtx.bc4_encode(src, 0)}[fmt]()
+            j = job(op, fmt, w, h, 0, 0, params=p, chan0=0)
+        else:
+            src = rng.integers(0, 256, nb * bpb, dtype=np.uint8)
+            out = np.zeros(w * h * 4, np.uint8)
+            ref = {FMT_BC1: lambda: qtx.bc1_decode(src, w, h, 2, True), FMT_BC3: lambda: qtx.bc3_decode(src, w, h, 2),
+                   FMT_BC4: lambda: qtx.bc4_decode(src, w, h, 0)}[fmt]()
+            j = job(op, fmt, w, h, 0, 0, chan0=0, write_alpha=True, interpolator=2)
+        jobs.append((j, np.ascontiguousarray(src).reshape(-1), out))
+        want.append(np.asarray(ref).reshape(-1))
+    for _ in range(2):  # twice: the second run reuses the context's buffers
+        for _, _, o in jobs:
+            o[:] = 0  # as the plain calls' outputs start (a BC4 decode writes its channel only)
+        run_host_batch(qtx, jobs)
+        for k, ((_, _, o), w) in enumerate(zip(jobs, want)):
+            assert (o == w).all(), 'job %d %s differs' % (k, specs[k])
