@@ -951,6 +951,10 @@ def main():
                     help='only run 3 device-to-device copies of MIB MiB (calibrates the HBM byte counters) and exit')
     ap.add_argument('--level-probe', type=int, default=-1, metavar='LEVEL',
                     help='only run 3 BC1 encodes of the 4096x4096 texture at LEVEL (FourColor, Ideal; for counter runs) and exit')
+    ap.add_argument('--host-probe', type=int, default=-1, metavar='LEVEL',
+                    help='only run BC1 encodes of the 4096x4096 texture at LEVEL from pageable host memory through '
+                         'qtx_bc1_encode (ThreeColorBlack at 18, else FourColor; Ideal), in a process holding no other '
+                         'streams, print their median time and exit')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: the codec has no multi-GPU path (textures shard with no '
@@ -973,6 +977,21 @@ def main():
         for _ in range(3):
             dst.copy_from(src)
         print(json.dumps({'copy_probe_bytes': n, 'copies': 3}), flush=True)
+        return
+    if a.host_probe >= 0:
+        from quicktex_amd import synth
+        from quicktex_amd.capi import COLOR_FOUR, COLOR_THREE_BLACK
+        img = synth.mixture_rgba(W, H, seed=0x5EED0002, near_black=0.10)
+        p = q.params(a.host_probe, COLOR_THREE_BLACK if a.host_probe == 18 else COLOR_FOUR, 0)
+        q.bc1_encode(img, p)  # warm-up: staging and band resources, clocks
+        t = []
+        for _ in range(7):
+            t0 = time.perf_counter()
+            q.bc1_encode(img, p)
+            t.append(time.perf_counter() - t0)
+        ms = sorted(t)[3] * 1e3
+        print(json.dumps({'host_probe': a.host_probe, 'encodes': 7, 'median_ms': round(ms, 3),
+                          'mpx_s': round(W * H / ms / 1e3, 1)}), flush=True)
         return
     if a.level_probe >= 0:
         from quicktex_amd import synth
