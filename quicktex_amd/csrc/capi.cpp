@@ -222,6 +222,12 @@ hipError_t setup_bands(qtx_ctx *ctx) {
 
 }  // namespace
 
+namespace qtx {
+namespace rt {
+hipError_t ensure_copy_stream(qtx_ctx *ctx) { return setup_bands(ctx); }
+}  // namespace rt
+}  // namespace qtx
+
 extern "C" {
 
 const char *qtx_last_error(void) { return last_error(); }

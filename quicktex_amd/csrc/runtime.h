@@ -63,6 +63,10 @@ struct qtx_ctx {
 namespace qtx {
 namespace rt {
 
+// Makes ctx->copy_stream (and the band events) if it is not there yet; under ctx->mu,
+// the device current.
+hipError_t ensure_copy_stream(qtx_ctx *ctx);
+
 // Thread-local message for qtx_last_error(); returns st.
 qtx_status fail(qtx_status st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 qtx_status hip_fail(hipError_t e, const char *what);

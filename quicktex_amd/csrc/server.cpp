@@ -92,7 +92,8 @@ struct qtx_server {
     uint8_t *h_stage = nullptr;
     srv::Arena arena, stage;
     std::vector<Slot> slots;
-    // the two streams batches run on: slot i's copies and kernels on s_k[i % 2]
+    // the two streams batches run on: slot i's copies and kernels on s_k[i % 2];
+    // the context's own (its stream and copy stream), not made for the server
     hipStream_t s_k[2] = {nullptr, nullptr};
     // test hook: the batch with this number (1 = the first) fails before any of
     // its work is enqueued, as a call the runtime refuses would (0: never)
@@ -356,8 +357,6 @@ void destroy(qtx_server *s) {
         if (sl.d_queues) (void)hipFree(sl.d_queues);
         if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
     }
-    for (hipStream_t st : s->s_k)
-        if (st) (void)hipStreamDestroy(st);
     if (s->d_arena) (void)hipFree(s->d_arena);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     delete s;
@@ -395,11 +394,16 @@ qtx_status qtx_server_start(qtx_ctx *ctx, const qtx_server_config *config, qtx_s
     if (const char *f = std::getenv("QTX_SERVER_INJECT_FAULT")) s->inject_fault_batch = std::strtoull(f, nullptr, 10);
     hipError_t e = hipMalloc(&s->d_arena, c.device_bytes);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_stage, c.staging_bytes, hipHostMallocDefault);
-    // two streams: the runtime gives a process a few hardware queues (four by
-    // default) and lets further streams share them, and work on a shared queue runs
-    // in order, so the server keeps to two streams of its own
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k[0], hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_k[1], hipStreamNonBlocking);
+    // two streams, the context's: the runtime gives a process a few hardware queues
+    // (four by default) and lets further streams share them, and work on a shared
+    // queue runs in order -- two streams more than the context's could put the
+    // server's pair on one queue, and a small request behind a long batch
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        e = ensure_copy_stream(ctx);
+    }
+    s->s_k[0] = ctx->stream;
+    s->s_k[1] = ctx->copy_stream;
     s->slots.resize((size_t)c.slots);
     for (Slot &sl : s->slots) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev_k, hipEventDisableTiming);
