@@ -112,8 +112,13 @@ hipError_t Stager::init() {
     return hipSuccess;
 }
 
-hipError_t Stager::upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s) {
-    if (n < kMinBytes || host_pinned(src) || init() != hipSuccess)
+void Stager::host_copy(uint8_t *dst, const uint8_t *src, size_t n) {
+    if (n >= ((size_t)256 << 10) && init() == hipSuccess) pool_->copy(dst, src, n);
+    else std::memcpy(dst, src, n);
+}
+
+hipError_t Stager::upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s, size_t min_bytes) {
+    if (n < min_bytes || host_pinned(src) || init() != hipSuccess)
         return hipMemcpyAsync(d_dst, src, n, hipMemcpyHostToDevice, s);
     for (size_t off = 0; off < n; off += kChunk) {
         const size_t len = std::min(kChunk, n - off);

@@ -69,9 +69,9 @@ class Stager {
     static constexpr size_t kMinBytes = (size_t)8 << 20;  // smaller transfers take the runtime's path
     static constexpr int kWorkers = 3;
     ~Stager();
-    // Enqueues the copy of n bytes of host memory src to device memory d_dst on s.
-    // src is no longer read once it returns.
-    hipError_t upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s);
+    // Enqueues the copy of n bytes of host memory src to device memory d_dst on s,
+    // staged from min_bytes on.  src is no longer read once it returns.
+    hipError_t upload(uint8_t *d_dst, const uint8_t *src, size_t n, hipStream_t s, size_t min_bytes = kMinBytes);
     // Copies n bytes of device memory d_src to host memory dst after the work
     // already on s.  A staged copy (min_bytes and more, pageable dst) is complete
     // on return; any other is enqueued on s (dst holds the bytes once s is
@@ -87,6 +87,9 @@ class Stager {
     // kPartsMinBytes, or with every part pinned, each part moves on its own
     // (enqueued on s).
     static constexpr size_t kPartsMinBytes = (size_t)1 << 20;
+    // memcpy between host buffers on the copy threads (large ones; small ones on the
+    // caller's thread alone).
+    void host_copy(uint8_t *dst, const uint8_t *src, size_t n);
     hipError_t upload_parts(uint8_t *d_dst, const HostPart *parts, size_t nparts, size_t n, hipStream_t s);
     hipError_t download_parts(const HostPart *parts, size_t nparts, const uint8_t *d_src, size_t n, hipStream_t s);
 
