@@ -14,7 +14,10 @@
  *   - `*_device` entry points take device pointers and a hipStream_t passed as
  *     `void*` (NULL = the null stream); they enqueue work and return without
  *     synchronising.  The host entry points copy host buffers through the
- *     context's staging memory and return after the result is back on the host.
+ *     context's staging memory (pageable ones of 8 MiB and more through pinned
+ *     chunks; pinned ones directly) and return after the result is back on the
+ *     host; BC1/BC3 encodes of 16 MiB and more upload in row bands that overlap
+ *     the encode.  The results are those of the device entry points.
  *   - Nothing is thrown across the ABI.  On failure a status is returned and
  *     qtx_last_error() (thread-local) describes it.
  */
@@ -334,8 +337,8 @@ typedef struct qtx_server_stats {
 
 typedef struct qtx_server qtx_server;
 /* Starts the loop on ctx's device (config may be NULL for the defaults).
- * Batches run on two streams of the server's own, consecutive batches on
- * alternate ones, so that their kernels overlap; within a batch, decodes and
+ * Batches run on the context's two streams (its stream and copy stream),
+ * consecutive batches on alternate ones, so that their kernels overlap; within a batch, decodes and
  * BC4/BC5 run before BC1/BC3 work.  A batch whose launch the runtime refuses fails
  * with QTX_EDEVICE and the server goes on; an error raised by work on the device
  * fails everything in flight and waiting, and later submits return QTX_EDEVICE. */
