@@ -138,14 +138,15 @@ hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStr
     hipError_t e;
     if (n < min_bytes || host_pinned(dst) || init() != hipSuccess)
         return hipMemcpyAsync(dst, d_src, n, hipMemcpyDeviceToHost, s);
-    const size_t chunks = (n + kChunk - 1) / kChunk;
+    const size_t C = chunk_for(n);
+    const size_t chunks = (n + C - 1) / C;
     const unsigned first = next_;
     next_ += (unsigned)chunks;
     // chunk k goes to slot (first + k) % kSlots; the DMAs of the next kSlots - 1
     // chunks run while chunk k is copied out
     auto slot = [&](size_t k) { return (unsigned)(first + k) % kSlots; };
     auto enqueue = [&](size_t k) -> hipError_t {
-        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        const size_t off = k * C, len = std::min(C, n - off);
         // the slot's last chunk (an upload's DMA, or this call's chunk k - kSlots,
         // copied out already) is done with it
         hipError_t r = hipEventSynchronize(ev_[slot(k)]);
@@ -158,7 +159,7 @@ hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStr
     for (size_t k = 0; k < chunks; k++) {
         if ((e = hipEventSynchronize(ev_[slot(k)])) != hipSuccess) return e;
         if (k + kSlots - 1 < chunks && (e = enqueue(k + kSlots - 1)) != hipSuccess) return e;
-        const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+        const size_t off = k * C, len = std::min(C, n - off);
         pool_->copy(dst + off, pin_ + slot(k) * kChunk, len);
     }
     return hipSuccess;

@@ -66,7 +66,19 @@ class Stager {
     // engine); 16 MiB ones lengthen the pipeline's fill and drain.
     static constexpr int kSlots = 8;
     static constexpr size_t kChunk = (size_t)8 << 20;
-    static constexpr size_t kMinBytes = (size_t)8 << 20;  // smaller transfers take the runtime's path
+    static constexpr size_t kMinBytes = (size_t)8 << 20;  // smaller uploads take the runtime's path
+    // Downloads are staged from kMinDownBytes on, and one below four chunks moves in
+    // four smaller ones (at least kMinChunk), so that the copy-out of one overlaps the
+    // DMA of the next (decodes of 1448x1448: 0.63 -> 0.43 ms).  Uploads keep whole
+    // chunks: there the copy in comes before the DMA, and smaller ones doubled the
+    // time of a 4 MiB upload.
+    static constexpr size_t kMinDownBytes = (size_t)2 << 20;
+    static constexpr size_t kMinChunk = (size_t)512 << 10;
+    static size_t chunk_for(size_t n) {
+        if (n >= 4 * kChunk) return kChunk;
+        const size_t q = ((n / 4) + 65535) & ~(size_t)65535;
+        return q < kMinChunk ? kMinChunk : q;
+    }
     static constexpr int kWorkers = 3;
     ~Stager();
     // Enqueues the copy of n bytes of host memory src to device memory d_dst on s,
@@ -78,7 +90,7 @@ class Stager {
     // synchronised).
     // min_bytes: the size from which the copy is staged (kMinBytes, or less where
     // the caller moves one transfer in several parts).
-    hipError_t download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s, size_t min_bytes = kMinBytes);
+    hipError_t download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStream_t s, size_t min_bytes = kMinDownBytes);
     // The same for a device range whose pieces come from (upload) or go to
     // (download) several host buffers: the range [0, n) moves in chunks and each
     // chunk's pieces are copied to or from their buffers, so many small buffers

@@ -1,10 +1,11 @@
 """GPU: the host entry points' staging and row bands (quicktex_amd/csrc/staging.cpp,
 quicktex_amd/csrc/capi.cpp).
 
-Pageable host buffers of 8 MiB and more move through the context's eight pinned
-8 MiB chunks, a few threads copying one chunk while the DMA engine moves the ones
-before it; smaller ones take the runtime's own path, and pinned ones go to the
-DMA engine directly.  BC1 and BC3 host encodes of 16 MiB of texels and more are
+Pageable host buffers move through the context's eight pinned 8 MiB chunks, a few
+threads copying one chunk while the DMA engine moves the others: uploads of 8 MiB
+and more in whole chunks, downloads of 2 MiB and more (below 32 MiB in four smaller
+chunks); smaller ones take the runtime's own path, and pinned ones go to the DMA
+engine directly.  BC1 and BC3 host encodes of 16 MiB of texels and more are
 cut into bands of block rows, each encoded as soon as its rows have landed, on
 two streams in turn.  Checked here, byte for byte, against the device entry
 points on the same data: inputs just below and at the staging threshold, of
@@ -25,9 +26,11 @@ from quicktex_amd.capi import QtxError, _ptr
 
 pytestmark = pytest.mark.gpu
 
-# 8 MiB - 4 KiB, 8 MiB (one chunk), two chunks with a short last one, four ragged chunks,
-# nine (more chunks than slots: the first slot comes round again within the call)
-SIZES = [(1024, 2047), (1024, 2048), (2048, 1025), (2501, 3001), (4200, 4100)]
+# uploads: 8 MiB - 4 KiB, 8 MiB (one chunk), two chunks with a short last one, four ragged
+# chunks, nine (more chunks than slots: the first slot comes round again within the call);
+# downloads (the decodes' texels): 2 MiB - 2 KiB and 2 MiB (the threshold), and the same
+# sizes in quarter chunks (below 32 MiB) or whole ones
+SIZES = [(512, 1023), (512, 1024), (1024, 2047), (1024, 2048), (2048, 1025), (2501, 3001), (4200, 4100)]
 # banded BC1/BC3 encodes: 16 MiB exactly, ragged both ways, wide and short (65
 # block rows), tall and narrow
 BANDED = [(2048, 2048), (2050, 2047), (16384, 257), (1024, 4100)]
