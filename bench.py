@@ -703,25 +703,61 @@ def c0_leg(a, q, hip, np):
     return out
 
 
+# The last-level cache (256 MB of Infinity Cache) holds a texture of tens of MiB
+# from one launch to the next, so launches repeated over one buffer read part of
+# it from the cache, not from HBM (measured: BC4 encode of 4096x4096 0.0115 ms on
+# one buffer, 0.0132 ms rotating over five).  The HBM-bound legs rotate over
+# copies whose inputs and outputs together exceed the cache.
+ROTATE_BYTES = 768 << 20
+
+
+def _rotation(per_copy_bytes):
+    """Copies to rotate over so that they hold at least ROTATE_BYTES."""
+    return max(2, -(-ROTATE_BYTES // per_copy_bytes))
+
+
+def _cycler(fns):
+    """One callable that runs fns[0], fns[1], ... in turn."""
+    it = [0]
+
+    def step():
+        fns[it[0] % len(fns)]()
+        it[0] += 1
+    return step
+
+
 def c1_leg(a, q, hip, np):
     """BC4 (channel 0) and BC5 (channels 0, 1) encode and decode of a 2048x2048
-    texture of smoothed noise with 20% per-block constants."""
+    texture of smoothed noise with 20% per-block constants; each timed launch
+    reads a different copy of its input, so that the inputs come from HBM."""
     from quicktex_amd import synth
     S = 2048
     img = synth.smooth_channels(S, S, seed=0x5EED0001, nchan=2)
     nb = (S // 4) * (S // 4)
-    d_img, d_blk, d_out = hip.Buffer.from_array(img), hip.Buffer(nb * 16), hip.Buffer(S * S * 4)
+    K = _rotation(S * S * 4 * 2 + nb * 16)
+    d_img = [hip.Buffer.from_array(img) for _ in range(K)]
+    d_blk = [hip.Buffer(nb * 16) for _ in range(K)]
+    d_out = [hip.Buffer(S * S * 4) for _ in range(K)]
     stream = hip.Stream()
     sh = stream.handle
-    cases = [('bc4_encode', 4.5, lambda: q.bc4_encode_device(0, d_img.ptr, S, S, d_blk.ptr, sh)),
-             ('bc4_decode', 4.5, lambda: q.bc4_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
-             ('bc5_encode', 5.0, lambda: q.bc5_encode_device(0, 1, d_img.ptr, S, S, d_blk.ptr, sh)),
-             ('bc5_decode', 5.0, lambda: q.bc5_decode_device(0, 1, d_blk.ptr, S, S, d_out.ptr, sh))]
-    out = {'workload': '2048x2048, smoothed noise + 20%% per-block constants; BC4 channel 0, BC5 channels 0, 1'}
+
+    def per_copy(call):
+        return _cycler([lambda k=k: call(k) for k in range(K)])
+    enc4 = lambda k: q.bc4_encode_device(0, d_img[k].ptr, S, S, d_blk[k].ptr, sh)  # noqa: E731
+    enc5 = lambda k: q.bc5_encode_device(0, 1, d_img[k].ptr, S, S, d_blk[k].ptr, sh)  # noqa: E731
+    cases = [('bc4_encode', 4.5, enc4, None),
+             ('bc4_decode', 4.5, lambda k: q.bc4_decode_device(0, d_blk[k].ptr, S, S, d_out[k].ptr, sh), enc4),
+             ('bc5_encode', 5.0, enc5, None),
+             ('bc5_decode', 5.0, lambda k: q.bc5_decode_device(0, 1, d_blk[k].ptr, S, S, d_out[k].ptr, sh), enc5)]
+    out = {'workload': '2048x2048, smoothed noise + 20%% per-block constants; BC4 channel 0, BC5 channels 0, 1',
+           'rotation': '%d copies of input and output (%d MiB), one per timed launch in turn' % (
+               K, K * (S * S * 8 + nb * 16) >> 20)}
     lib, kind = cpu_lib() if not a.no_cpu else (None, None)
-    b4 = b5 = None
-    for name, bpp, fn in cases:
-        ms = _events_ms(hip, stream, fn, 20)
+    for name, bpp, call, producer in cases:
+        if producer is not None:  # each copy's blocks from its own encode
+            for k in range(K):
+                producer(k)
+        ms = _events_ms(hip, stream, per_copy(call), 4 * K)
         gbs = S * S * bpp / (ms / 1e3) / 1e9
         r = {'kernel_ms': round(ms, 4), 'device_mpx_s': round(S * S / ms / 1e3, 1),
              'roofline': {'bound': 'hbm', 'unit': 'GB/s', 'achieved': round(gbs, 1), 'peak': 8000.0,
@@ -732,22 +768,16 @@ def c1_leg(a, q, hip, np):
             elif name == 'bc5_encode':
                 cfn = lambda: lib.bc5_encode(img, 0, 1)  # noqa: E731
             elif name == 'bc4_decode':
-                cfn = lambda: lib.bc4_decode(b4, S, S, 0)  # noqa: E731
+                blocks0 = d_blk[0].download(nb * 8)
+                cfn = lambda: lib.bc4_decode(blocks0, S, S, 0)  # noqa: E731
             else:
-                cfn = lambda: lib.bc5_decode(b5, S, S, 0, 1)  # noqa: E731
-            if name == 'bc4_decode':
-                d_blk.upload(b4)
-            if name == 'bc5_decode':
-                d_blk.upload(b5)
-            fn()
+                blocks0 = d_blk[0].download(nb * 16)
+                cfn = lambda: lib.bc5_decode(blocks0, S, S, 0, 1)  # noqa: E731
+            call(0)
             stream.synchronize()
-            got = (d_blk.download(nb * (8 if '4' in name else 16)) if 'encode' in name else d_out.download())
+            got = d_blk[0].download(nb * (8 if '4' in name else 16)) if 'encode' in name else d_out[0].download()
             secs = _median_s(cfn, 5)
             want = cfn()
-            if name == 'bc4_encode':
-                b4 = want
-            if name == 'bc5_encode':
-                b5 = want
             r['cpu_baseline'] = {'value': round(S * S / secs / 1e6, 3), 'unit': 'Mpixel/s', 'cores': 1,
                                  'kind': kind, 'cpu_model': cpu_model(),
                                  'sample': 'the whole texture, median of 5 (serial, as the reference runs it)',
@@ -802,18 +832,38 @@ def sweep_leg(a, q, hip, np):
         log('level %2d: %.3f ms, %.0f Mpx/s, %.0f ops/block, VALU %.1f%%' % (
             lvl, ms, S * S / ms / 1e3, ops, 100 * tops / VALU_PEAK_TOPS))
     hbm = {}
-    cases = [('bc4_encode', 4.5, lambda: q.bc4_encode_device(0, d_sm.ptr, S, S, d_blk.ptr, sh)),
-             ('bc5_encode', 5.0, lambda: q.bc5_encode_device(0, 1, d_sm.ptr, S, S, d_blk.ptr, sh)),
-             ('bc1_decode', 4.5, lambda: q.bc1_decode_device(0, 0, d_blk.ptr, S, S, d_out.ptr, sh)),
-             ('bc3_decode', 5.0, lambda: q.bc3_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
-             ('bc4_decode', 4.5, lambda: q.bc4_decode_device(0, d_blk.ptr, S, S, d_out.ptr, sh)),
-             ('bc5_decode', 5.0, lambda: q.bc5_decode_device(0, 1, d_blk.ptr, S, S, d_out.ptr, sh))]
-    for name, bpp, fn in cases:
-        ms = timeit(fn, 10)
+    # each timed launch reads a different copy of its input (and writes a different
+    # output), so that the inputs come from HBM and not from the last-level cache
+    K = _rotation(S * S * 4 * 2 + nb * 16)
+    d_sms = [d_sm] + [hip.Buffer.from_array(sm) for _ in range(K - 1)]
+    d_blks = [d_blk] + [hip.Buffer(nb * 16) for _ in range(K - 1)]
+    d_outs = [d_out] + [hip.Buffer(S * S * 4) for _ in range(K - 1)]
+    enc4 = lambda k: q.bc4_encode_device(0, d_sms[k].ptr, S, S, d_blks[k].ptr, sh)  # noqa: E731
+    enc5 = lambda k: q.bc5_encode_device(0, 1, d_sms[k].ptr, S, S, d_blks[k].ptr, sh)  # noqa: E731
+    bc1_blocks = lambda k: q.bc1_encode_device(q.params(5, COLOR_FOUR, 0), d_sms[k].ptr, S, S,  # noqa: E731
+                                               d_blks[k].ptr, sh)
+    bc3_blocks = lambda k: q.bc3_encode_device(q.params(5, COLOR_FOUR, 0), d_sms[k].ptr, S, S,  # noqa: E731
+                                               d_blks[k].ptr, sh)
+    cases = [('bc4_encode', 4.5, enc4, None),
+             ('bc5_encode', 5.0, enc5, None),
+             ('bc1_decode', 4.5, lambda k: q.bc1_decode_device(0, 0, d_blks[k].ptr, S, S, d_outs[k].ptr, sh),
+              bc1_blocks),
+             ('bc3_decode', 5.0, lambda k: q.bc3_decode_device(0, d_blks[k].ptr, S, S, d_outs[k].ptr, sh),
+              bc3_blocks),
+             ('bc4_decode', 4.5, lambda k: q.bc4_decode_device(0, d_blks[k].ptr, S, S, d_outs[k].ptr, sh), enc4),
+             ('bc5_decode', 5.0, lambda k: q.bc5_decode_device(0, 1, d_blks[k].ptr, S, S, d_outs[k].ptr, sh),
+              enc5)]
+    for name, bpp, call, producer in cases:
+        if producer is not None:  # each copy's blocks from an encode of its own
+            for k in range(K):
+                producer(k)
+        ms = timeit(_cycler([lambda k=k: call(k) for k in range(K)]), 4 * K)
         gbs = S * S * bpp / (ms / 1e3) / 1e9
         hbm[name] = {'ms': round(ms, 4), 'mpx_s': round(S * S / ms / 1e3, 1), 'gb_s': round(gbs, 1),
                      'hbm_frac': round(gbs / 8000.0, 4)}
         log('%s: %.4f ms, %.0f GB/s (%.1f%% of 8 TB/s)' % (name, ms, gbs, gbs / 80.0))
+    hbm['rotation'] = '%d copies of input and output (%d MiB), one per timed launch in turn' % (
+        K, K * (S * S * 8 + nb * 16) >> 20)
     stream.destroy()
     return {'size': '%dx%d' % (S, S), 'bc1_levels_fourcolor_ideal': levels, 'hbm_bound': hbm}
 
