@@ -94,8 +94,9 @@ class Stager {
     // The same for a device range whose pieces come from (upload) or go to
     // (download) several host buffers: the range [0, n) moves in chunks and each
     // chunk's pieces are copied to or from their buffers, so many small buffers
-    // cost one staged transfer (bytes of the range no part covers are left as they
-    // are / not copied out).  Parts ascend by offset and do not overlap.  Below
+    // cost one staged transfer.  Device bytes of the range that no part covers (the
+    // callers' alignment gaps) receive unspecified values on the way up and are
+    // not copied out on the way down.  Parts ascend by offset and do not overlap.  Below
     // kPartsMinBytes, or with every part pinned, each part moves on its own
     // (enqueued on s).
     static constexpr size_t kPartsMinBytes = (size_t)1 << 20;
