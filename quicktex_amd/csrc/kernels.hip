@@ -181,9 +181,17 @@ __device__ __forceinline__ void decode_block(const uint8_t *__restrict__ blocks,
             bc4_decode_into(px, ab.x & 0xFFu, (ab.x >> 8) & 0xFFu, (uint64_t)(ab.x >> 16) | ((uint64_t)ab.y << 16), 24u);
     } else {
         const int nch = KIND == DEC_BC5 ? 2 : 1;
+        uint2 qs[2];
+        if (KIND == DEC_BC5) {  // both channels' blocks in one 16-byte load
+            const uint4 v = reinterpret_cast<const uint4 *>(blocks)[gid];
+            qs[0] = make_uint2(v.x, v.y);
+            qs[1] = make_uint2(v.z, v.w);
+        } else {
+            qs[0] = reinterpret_cast<const uint2 *>(blocks)[gid];
+        }
 #pragma unroll
         for (int k = 0; k < nch; k++) {
-            const uint2 q = reinterpret_cast<const uint2 *>(blocks)[gid * nch + k];
+            const uint2 q = qs[k];
             const int ch = k == 0 ? p0 : p1;
             bc4_decode_into(px, q.x & 0xFFu, (q.x >> 8) & 0xFFu, (uint64_t)(q.x >> 16) | ((uint64_t)q.y << 16),
                             8u * (uint32_t)ch);
