@@ -165,6 +165,16 @@ hipError_t Stager::download(uint8_t *dst, const uint8_t *d_src, size_t n, hipStr
     return hipSuccess;
 }
 
+hipError_t Stager::direct(size_t n, uint8_t **p) {
+    if (n > kSlots * kChunk) return hipErrorInvalidValue;
+    hipError_t e = init();
+    if (e != hipSuccess) return e;
+    for (size_t k = 0; k * kChunk < n; k++)
+        if ((e = hipEventSynchronize(ev_[k])) != hipSuccess) return e;
+    *p = pin_;
+    return hipSuccess;
+}
+
 namespace {
 // Copies between a chunk [c0, c0 + len) of the range and the parts it overlaps:
 // to the parts (out) or from them into the chunk.

@@ -105,6 +105,10 @@ class Stager {
     void host_copy(uint8_t *dst, const uint8_t *src, size_t n);
     hipError_t upload_parts(uint8_t *d_dst, const HostPart *parts, size_t nparts, size_t n, hipStream_t s);
     hipError_t download_parts(const HostPart *parts, size_t nparts, const uint8_t *d_src, size_t n, hipStream_t s);
+    // The first n bytes (at most kSlots * kChunk) of the pinned chunks, for a call
+    // whose kernels read and write host memory in place; returns once the chunks'
+    // earlier transfers are done.  They are the caller's until its next call here.
+    hipError_t direct(size_t n, uint8_t **p);
 
   private:
     hipError_t init();

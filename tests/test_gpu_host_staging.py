@@ -189,3 +189,69 @@ def test_threads_and_contexts(qtx):
         for q in own:
             q.close()
     assert not errors, errors
+
+
+def _device_call(q, hip, fn, src, out_bytes):
+    """Runs a device entry point fn(d_src, d_out) on copies of src; returns its output bytes."""
+    d_in, d_out = hip.Buffer.from_array(np.ascontiguousarray(src)), hip.Buffer(out_bytes)
+    fn(d_in.ptr, d_out.ptr)
+    hip.synchronize()
+    return d_out.download().reshape(-1)
+
+
+# Host calls on pageable buffers whose input and output take at most 1.5 MiB together run
+# their kernels on the context's pinned host memory (no DMA transfer): one block, ragged, small, and either side of the
+# limit -- BC1 encode and decode 590x590 in place, 592x592 through the DMA engine; BC3/BC5
+# encode 560x560 in place, 590x590 through the DMA engine
+DIRECT = [(1, 1), (5, 3), (130, 66), (560, 560), (590, 590), (592, 592)]
+
+
+@pytest.mark.parametrize('w,h', DIRECT)
+def test_direct_calls(qtx, w, h):
+    """Every host entry point, against its device entry point on the same data."""
+    from quicktex_amd import hip
+    print('direct %dx%d' % (w, h), flush=True)
+    img = synth.mixture_rgba(w, h, seed=0xD1 + w * 7 + h, near_black=0.1)
+    nb = _nb(w, h)
+    p5, p10 = qtx.params(5, 4, 0), qtx.params(10, 4, 2)
+    dev = lambda fn, src, n: _device_call(qtx, hip, fn, src, n)  # noqa: E731
+    assert (qtx.bc1_encode(img, p5) == dev(lambda i, o: qtx.bc1_encode_device(p5, i, w, h, o), img, nb * 8)).all()
+    assert (qtx.bc3_encode(img, p10) == dev(lambda i, o: qtx.bc3_encode_device(p10, i, w, h, o), img, nb * 16)).all()
+    assert (qtx.bc4_encode(img, 1) == dev(lambda i, o: qtx.bc4_encode_device(1, i, w, h, o), img, nb * 8)).all()
+    assert (qtx.bc5_encode(img, 0, 2) == dev(lambda i, o: qtx.bc5_encode_device(0, 2, i, w, h, o), img, nb * 16)).all()
+    rng = np.random.default_rng(w * 1000 + h)
+    b8, b16 = rng.integers(0, 256, nb * 8, dtype=np.uint8), rng.integers(0, 256, nb * 16, dtype=np.uint8)
+    px = w * h * 4
+    assert (qtx.bc1_decode(b8, w, h, 3, True).reshape(-1)
+            == dev(lambda i, o: qtx.bc1_decode_device(3, 1, i, w, h, o), b8, px)).all()
+    assert (qtx.bc3_decode(b16, w, h, 1).reshape(-1) == dev(lambda i, o: qtx.bc3_decode_device(1, i, w, h, o), b16, px)).all()
+    assert (qtx.bc4_decode(b8, w, h, 2).reshape(-1) == dev(lambda i, o: qtx.bc4_decode_device(2, i, w, h, o), b8, px)).all()
+    assert (qtx.bc5_decode(b16, w, h, 1, 3).reshape(-1)
+            == dev(lambda i, o: qtx.bc5_decode_device(1, 3, i, w, h, o), b16, px)).all()
+
+
+def test_direct_error_then_ok(qtx):
+    """A launch refused on the in-place path fails the call; the context works on."""
+    from quicktex_amd import hip
+    img = synth.mixture_rgba(64, 48, seed=0xD2, near_black=0.1)
+    p = qtx.params(5, 4, 0)
+    p.interpolator = 9
+    with pytest.raises(QtxError):
+        qtx.bc1_encode(img, p)
+    p = qtx.params(5, 4, 0)
+    assert (qtx.bc1_encode(img, p) == _device_colour(qtx, hip, img, p, False)).all()
+
+
+def test_direct_pinned(qtx):
+    """Pinned input and output below the limit: moved by the DMA engine, not in place."""
+    from quicktex_amd import hip
+    w, h = 200, 150
+    img = synth.mixture_rgba(w, h, seed=0xD3, near_black=0.1)
+    p = qtx.params(9, 4, 3)
+    want = _device_colour(qtx, hip, img, p, True)
+    src, dst = hip.Pinned(img.nbytes), hip.Pinned(want.nbytes)
+    src.array[:] = img.reshape(-1)
+    dst.array[:] = 0xA5
+    qtx.check(qtx.lib.qtx_bc3_encode(qtx.ctx, ctypes.byref(p), _ptr(src.array), w, h, _ptr(dst.array), dst.nbytes))
+    n, first = _first_bad(dst.array, want, 16)
+    assert n == 0, '%d blocks differ, first %s' % (n, first)
