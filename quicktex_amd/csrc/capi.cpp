@@ -495,9 +495,9 @@ qtx_status qtx_bc5_decode_device(qtx_ctx *ctx, uint32_t chan0, uint32_t chan1, c
 namespace {
 
 // Host calls on pageable buffers whose input and output together take at most
-// this many bytes skip the DMA engine: the input is copied into the context's pinned chunks, the
-// kernels read it and write their output there over the link, and the output is
-// copied out.  A transfer by the DMA engine costs 8-15 us of set-up and wait
+// this many bytes skip the DMA engine: the input is copied into the context's
+// pinned chunks, the kernels read it and write their output there over the link,
+// and the output is copied out.  A transfer by the DMA engine costs 8-15 us of set-up and wait
 // however small (measured: 256 KiB up 12-15 us, 32 KiB down 8-9 us, a launch and
 // its wait 11 us), which is most of a small call: host calls of 128x128 took
 // 19-50 us this way against 60-93 us through the DMA engine, 256x256 25-61 against
