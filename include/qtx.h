@@ -15,9 +15,10 @@
  *     `void*` (NULL = the null stream); they enqueue work and return without
  *     synchronising.  The host entry points copy host buffers through the
  *     context's staging memory (pageable uploads of 8 MiB and more and downloads
- *     of 2 MiB and more through pinned chunks; pinned buffers directly; pageable
- *     calls of at most 1.5 MiB in and out together run their kernels on the
- *     pinned chunks in place, QTX_HOST_DIRECT_MAX bytes to change it) and return
+ *     of 2 MiB and more through pinned chunks; pinned buffers directly; calls of
+ *     at most 1.5 MiB in and out together run their kernels in place, on the
+ *     caller's buffers if both are pinned, else on the pinned chunks if both are
+ *     pageable, QTX_HOST_DIRECT_MAX bytes to change it) and return
  *     after the result is back on the host; BC1/BC3 encodes of 16 MiB and more
  *     upload in row bands that overlap the encode.  The results are those of the
  *     device entry points.

@@ -528,13 +528,22 @@ qtx_status host_call(qtx_ctx *ctx, const uint8_t *in, size_t in_bytes, uint8_t *
     std::lock_guard<std::mutex> lk(ctx->mu);
     QTX_HIP(hipSetDevice(ctx->device));
     const size_t in_span = (in_bytes + 255) & ~(size_t)255;  // the output starts 256-byte aligned
+    const bool small = in_span + out_need <= direct_max_bytes();
+    const bool in_pinned = small && qtx::rt::host_pinned(in), out_pinned = small && qtx::rt::host_pinned(out);
+    if (in_pinned && out_pinned) {  // the kernels read and write the caller's pinned buffers
+        const qtx_status st = launch(in, out, (void *)ctx->stream);
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (st != QTX_OK) return st;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        return QTX_OK;
+    }
     uint8_t *pin = nullptr;
-    // pageable buffers only: pinned ones go to the DMA engine directly, which beats a
-    // copy through the chunks (the served mix from pinned buffers: small decodes 0.06
-    // ms by DMA against 0.09-0.12 ms in place, and the first write into a caller's
-    // untouched pinned output took 0.4-0.5 ms more)
-    if (in_span + out_need <= direct_max_bytes() && !qtx::rt::host_pinned(in) && !qtx::rt::host_pinned(out) &&
-        ctx->stager.direct(in_span + out_need, &pin) == hipSuccess) {
+    // both buffers pageable: a pinned one beside a pageable one goes to the DMA engine
+    // directly, which beats a copy through the chunks (the served mix from pinned
+    // buffers: small decodes 0.06 ms by DMA against 0.09-0.12 ms through the chunks,
+    // and the CPU's first write into a caller's untouched pinned output took 0.4-0.5
+    // ms more)
+    if (small && !in_pinned && !out_pinned && ctx->stager.direct(in_span + out_need, &pin) == hipSuccess) {
         std::memcpy(pin, in, in_bytes);
         const qtx_status st = launch(pin, pin + in_span, (void *)ctx->stream);
         const hipError_t e = hipStreamSynchronize(ctx->stream);

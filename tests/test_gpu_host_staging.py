@@ -243,7 +243,7 @@ def test_direct_error_then_ok(qtx):
 
 
 def test_direct_pinned(qtx):
-    """Pinned input and output below the limit: moved by the DMA engine, not in place."""
+    """Pinned input and output below the limit: the kernels read and write them in place."""
     from quicktex_amd import hip
     w, h = 200, 150
     img = synth.mixture_rgba(w, h, seed=0xD3, near_black=0.1)
@@ -255,3 +255,31 @@ def test_direct_pinned(qtx):
     qtx.check(qtx.lib.qtx_bc3_encode(qtx.ctx, ctypes.byref(p), _ptr(src.array), w, h, _ptr(dst.array), dst.nbytes))
     n, first = _first_bad(dst.array, want, 16)
     assert n == 0, '%d blocks differ, first %s' % (n, first)
+
+
+@pytest.mark.parametrize('pin_src,pin_dst', [(True, True), (True, False), (False, True)])
+def test_direct_pinned_decode(qtx, pin_src, pin_dst):
+    """Small BC1 / BC5 decodes from and to pinned buffers (in place when both are), twice in a row."""
+    from quicktex_amd import hip
+    w, h = 300, 222
+    nb = _nb(w, h)
+    rng = np.random.default_rng(0xD4)
+    for fmt, bpb in ((1, 8), (5, 16)):
+        blocks = rng.integers(0, 256, nb * bpb, dtype=np.uint8)
+        if fmt == 1:
+            want = _device_call(qtx, hip, lambda i, o: qtx.bc1_decode_device(2, 1, i, w, h, o), blocks, w * h * 4)
+        else:
+            want = _device_call(qtx, hip, lambda i, o: qtx.bc5_decode_device(3, 0, i, w, h, o), blocks, w * h * 4)
+        src = hip.Pinned(blocks.nbytes) if pin_src else None
+        dst = hip.Pinned(w * h * 4) if pin_dst else None
+        s_arr = src.array if src else blocks.copy()
+        d_arr = dst.array if dst else np.zeros(w * h * 4, np.uint8)
+        if src:
+            s_arr[:] = blocks
+        for _ in range(2):
+            d_arr[:] = 0x5A
+            if fmt == 1:
+                qtx.check(qtx.lib.qtx_bc1_decode(qtx.ctx, 2, 1, _ptr(s_arr), w, h, _ptr(d_arr), d_arr.size))
+            else:
+                qtx.check(qtx.lib.qtx_bc5_decode(qtx.ctx, 3, 0, _ptr(s_arr), w, h, _ptr(d_arr), d_arr.size))
+            assert (d_arr == want).all()
